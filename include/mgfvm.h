@@ -1,0 +1,173 @@
+/*
+ * mgfvm.h -- C ABI of the MI355X-native SIMPLE / FAS-multigrid steady-state
+ * solver for incompressible channel flow with a passive scalar on a
+ * staggered structured 2-D grid with obstacle masks.
+ *
+ * Drop-in boundary.  The reference (akashu4295/Multigrid-FVM,
+ * Version-6-26-04.c, "V6" below) has no library interface: its hot path is a
+ * set of internal C routines taking `struct SoA*` levels and mutable
+ * file-scope globals, driven by main().  Each entry point here replaces one
+ * group of those call sites; the citation says which.  Conventions:
+ *   - plain C types only; status 0 = success, negative = error;
+ *   - the library never calls exit(); the message of the last error is
+ *     available from mgf_last_error() (per solver) or mgf_error_string()
+ *     (calls without a solver, per thread);
+ *   - the library owns device memory, the caller owns host buffers;
+ *   - one solver per host thread at a time (externally synchronised); each
+ *     solver owns one HIP stream on the device it was created on.
+ *
+ * Grid conventions (V6 Appendix-A of the reference's own indexing): level ig
+ * has npx = nfx/2^ig + 2 by npy = nfy/2^ig + 2 points, row-major
+ * ij = j*npx + i.  u[ij] lives on the east face of cell (i,j), v[ij] on its
+ * north face; p, pp (pressure correction) and sc are cell centred; ghost
+ * layers are i = 0, npx-1 and j = 0, npy-1.
+ */
+#ifndef MGFVM_H
+#define MGFVM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGF_ABI_VERSION 1
+
+/* status codes */
+#define MGF_OK               0
+#define MGF_ERR_ARG         -1   /* bad argument (null pointer, level out of range, unknown name) */
+#define MGF_ERR_IO          -2   /* file cannot be opened (V6:282-286, 2369-2372 exit there) */
+#define MGF_ERR_PARSE       -3   /* malformed InputData / pattern.txt (V6:288-392, 2374-2390) */
+#define MGF_ERR_CONFIG      -4   /* inconsistent configuration (e.g. ig_obs > ngrid, V6:471 UB) */
+#define MGF_ERR_GEOMETRY    -5   /* obstacle/inlet outside the interior, or grid not 2:1 nested */
+#define MGF_ERR_LIMIT       -6   /* a size limit of this implementation is exceeded */
+#define MGF_ERR_DEVICE      -7   /* HIP runtime error */
+#define MGF_ERR_NONFINITE   -8   /* non-finite residual (the reference does not detect it) */
+#define MGF_ERR_UNSUPPORTED -9   /* option not supported by this build */
+
+/* Everything InputData and pattern.txt carry (V6:27-39 globals, grammar
+ * README.md:7-78).  Arrays are owned by whoever built the struct; the
+ * solver copies what it needs at mgf_create(). */
+typedef struct mgf_problem {
+    int nfx, nfy;              /* fine-grid cells in x and y */
+    int ngrid;                 /* multigrid levels */
+    int ig_obs;                /* 1-based level on which obstacles are given */
+    int ibeg_obs;              /* i offset of the obstacle pattern */
+    int ig_inlt, ig_outlt;     /* 1-based levels of discrete inlets/outlets (0: none) */
+    int nvcycle;               /* V-cycle limit */
+    int nswpm, nswpp;          /* smoother sweeps: momentum+scalar, pressure correction */
+    int isolve;                /* 1 Jacobi, 2 red-black Gauss-Seidel (V6:701-736) */
+    double omega;              /* read but unused by the reference (V6:1241) */
+    double convergence;        /* stop when the level-0 sum |div| falls below this */
+    double xl, yl;             /* domain size */
+    double amuf;               /* viscosity (1/Re) */
+    double prsc;               /* scalar Prandtl/Schmidt number */
+    double relxm, relxp;       /* under-relaxation: momentum, pressure */
+    int ibc[4];                /* boundary types xm, xp, ym, yp: 1 Dirichlet ghosts,
+                                  2 p' Dirichlet row, 3 periodic, 4 (xp) outflow */
+    double bc[4][4];           /* ghost values [xm,xp,ym,yp][u,v,p,sc] */
+    double guess[4];           /* initial u, v, p, sc */
+    double sc_obs;             /* scalar value imposed in obstacles */
+    int nobs;                  /* obstacle rectangles */
+    const int *obs;            /* nobs x {if, il, jf, jl}, 1-based on level ig_obs-1 */
+    int ninlt, noutlt;         /* discrete inlets/outlets */
+    const int *inlt_j, *outlt_j;        /* n x {jf, jl} */
+    const double *inlt_v, *outlt_v;     /* n x {u, v, sc} */
+    double dpdxm, dpdym;       /* body force; never assigned by the reference (0) */
+    const double *inlet_u;     /* optional: nfy level-0 inlet u values (rows 1..nfy),
+                                  applied after the initial state (extension) */
+} mgf_problem;
+
+/* precision modes */
+#define MGF_PREC_FP64   0      /* bit-exact with the reference CPU build */
+#define MGF_PREC_MIXED  1      /* FP32 smoothers/transfers, FP64 outer residuals */
+
+typedef struct mgf_options {
+    int precision;             /* MGF_PREC_* */
+    int use_graph;             /* capture one V-cycle in a hipGraph and replay it */
+    int device;                /* HIP device ordinal (-1: current) */
+    int strict_limits;         /* reject inputs beyond the reference's static limits
+                                  (nobs <= 1000, ninlt/noutlt <= 10, nvcycle <= 3000) */
+} mgf_options;
+
+typedef struct mgf_solver mgf_solver;
+
+/* ---- problem description (host only, no device needed) ---------------- */
+
+/* Read InputData + pattern.txt.  Replaces readInput (V6:280-401) and
+ * read_obstacle_data (V6:2367-2400); errors return a status instead of
+ * exit().  Free with mgf_problem_free. */
+int mgf_problem_read(const char *inputdata_path, const char *pattern_path,
+                     mgf_problem **out);
+void mgf_problem_free(mgf_problem *p);
+void mgf_options_default(mgf_options *o);
+const char *mgf_error_string(void);
+int mgf_abi_version(void);
+
+/* ---- solver lifetime -------------------------------------------------- */
+
+/* Build the level hierarchy, initial/boundary state and coarse geometry on
+ * the host exactly as AllocateMemorySoA (V6:402-506), init (V6:546-676) and
+ * rest_geom (V6:1391-1473) do, derive the device masks, and upload.
+ * Replaces V6:184-192 (allocation, setup and the OpenACC enter-data). */
+int mgf_create(const mgf_problem *p, const mgf_options *opt, mgf_solver **out);
+/* Replaces V6:270-273 (exit-data and free). */
+void mgf_destroy(mgf_solver *s);
+const char *mgf_last_error(const mgf_solver *s);
+int mgf_num_levels(const mgf_solver *s);
+int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy);
+/* the solver's HIP stream (a hipStream_t), for callers that order work on it */
+void *mgf_stream(mgf_solver *s);
+
+/* ---- reference-granular steps (parity and debugging) ------------------ */
+
+/* One SIMPLE relaxation of level ig: fluxes, solve_u, solve_v,
+ * solve_pprime, pbound, then outflow+solve_sc (leg 0, down-leg order
+ * V6:201-207) or solve_sc+outflow (leg 1, up-leg order V6:237-243).
+ * *mass_err (may be NULL) receives the level's sum |div| (V6:1096-1125). */
+int mgf_relax(mgf_solver *s, int ig, int leg, double *mass_err);
+/* FAS restriction ig -> ig+1: restv, fluxes x2, restru, restrv, restrsc
+ * (V6:211-220). */
+int mgf_restrict(mgf_solver *s, int ig_fine);
+/* Prolongation ig_coarse -> ig_coarse-1: prolu, prolv, prolp, prolsc
+ * (V6:230-233). */
+int mgf_prolong(mgf_solver *s, int ig_coarse);
+
+/* ---- whole cycles ------------------------------------------------------ */
+
+/* One V-cycle (V6:197-246); *mass_err = level-0 sum |div| recorded in it. */
+int mgf_vcycle(mgf_solver *s, int icyc, double *mass_err);
+/* V-cycles until the recorded residual is below tol or nvcycle cycles ran
+ * (V6:193-249).  history (may be NULL) receives nvcycle entries of which
+ * *ncycles are valid; *converged = 1 if the stopping test fired.  Returns
+ * MGF_ERR_NONFINITE (after finishing) if a recorded residual was not finite. */
+int mgf_solve(mgf_solver *s, int nvcycle, double tol, double *history,
+              int *ncycles, int *converged);
+
+/* ---- data ----------------------------------------------------------------- */
+
+/* Copy a level array to/from a caller buffer of npx*npy doubles.  Names:
+ * u v p pp sc resu resv ressc apu apv.  Replaces UpdateHostForOutput
+ * (V6:172-176) and gives the state access the reference's globals gave. */
+int mgf_get_field(mgf_solver *s, int ig, const char *name, double *out);
+int mgf_set_field(mgf_solver *s, int ig, const char *name, const double *in);
+int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
+                   double *pp, double *sc);
+
+/* ---- measurement ------------------------------------------------------------ */
+
+/* Kernel ids for mgf_time_kernel */
+#define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation */
+#define MGF_KERNEL_SWEEP_U   1   /* one full smoother sweep of the u equation */
+/* Time `reps` launches of one kernel of level ig with HIP events on the
+ * solver's stream, on the current state (the state is saved and restored).
+ * Returns the mean duration in microseconds and the algorithmic bytes one
+ * launch moves (8 words x word size x unknowns). */
+int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps,
+                    double *mean_us, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,740 @@
+/*
+ * mgf_kernels.hip -- FP64 kernels of the SIMPLE relaxation, smoothers and
+ * FAS grid transfers for gfx950 (MI355X).
+ *
+ * Arithmetic: every value is computed with the reference's expression tree
+ * (C left-to-right association), IEEE add/mul/div/fmax/fabs only, built with
+ * -ffp-contract=off so no FMA is formed.  The results are therefore
+ * bit-identical to the reference CPU build (Version-6-26-04.c compiled with
+ * -ffp-contract=off); only the order of the sum of |div| differs (a fixed
+ * tree here, so it is run-to-run deterministic).
+ *
+ * Mapping: one thread per grid point, 64x4 workgroups (one wavefront per
+ * 64-point row segment, so every array access of a wavefront is one
+ * contiguous 512-byte run of a row).  The work is HBM/MALL-bandwidth bound
+ * stencil code (about 0.16 flop per byte), so no matrix cores are used.
+ *
+ * Early exit: every kernel returns immediately once the device control
+ * block says the stopping test has fired, so V-cycles enqueued past
+ * convergence change nothing and the cycle count equals the reference's.
+ */
+#include <hip/hip_runtime.h>
+
+#include "mgf_dev.h"
+#include "mgf_plan.h"
+
+#define BX 64
+#define BY 4
+
+struct Coef {
+    double aw, ae, as, an, ap;
+};
+
+/* face fluxes from the materialised arrays (relaxation: cx/cy are computed
+ * once at its start, V6:201, and the u/v/sc assemblies read those) */
+struct FluxArr {
+    const double *cx, *cy;
+    __device__ double X(int k) const { return cx[k]; }
+    __device__ double Y(int k) const { return cy[k]; }
+};
+/* face fluxes recomputed from the current velocities (restriction: the
+ * reference recomputes them right before, V6:215-216, with nothing changing
+ * u/v in between), same expression as fluxes (V6:690-691) */
+struct FluxInl {
+    const double *u, *v;
+    double ax, ay;
+    __device__ double X(int k) const { return (0.5 * u[k]) * ax; }
+    __device__ double Y(int k) const { return (0.5 * v[k]) * ay; }
+};
+
+__device__ __forceinline__ bool ctl_done(const DevCtl *ctl) { return ctl && ctl->done; }
+
+/* coefu + no-slip wall doubling at one u face (V6:825-858) */
+template <class FX>
+__device__ __forceinline__ Coef coefu_at(const DevLevel &L, const FX &f, int i, int j)
+{
+    const int nx = L.npx;
+    const int ij = j * nx + i, ijs = ij - nx;
+    double cxm = 0.5 * (f.X(ij) + f.X(ij - 1));
+    double cxp = 0.5 * (f.X(ij) + f.X(ij + 1));
+    double cym = 0.5 * (f.Y(ijs) + f.Y(ijs + 1));
+    double cyp = 0.5 * (f.Y(ij) + f.Y(ij + 1));
+    Coef c;
+    c.aw = fmax(fabs(cxm), L.du_x) + cxm;
+    c.ae = fmax(fabs(cxp), L.du_x) - cxp;
+    c.as = fmax(fabs(cym), L.du_y) + cym;
+    c.an = fmax(fabs(cyp), L.du_y) - cyp;
+    c.ap = ((c.aw + c.ae) + c.as) + c.an;
+    if (j == L.npy - 2) { c.an += L.du_y; c.ap += L.du_y; }
+    if (j == 1) { c.as += L.du_y; c.ap += L.du_y; }
+    return c;
+}
+
+/* coefv + side-wall adds at one v face (V6:976-1015) */
+template <class FX>
+__device__ __forceinline__ Coef coefv_at(const DevLevel &L, const FX &f, int i, int j)
+{
+    const int nx = L.npx;
+    const int ij = j * nx + i, ijn = ij + nx, ijs = ij - nx;
+    double cxm = 0.5 * (f.X(ij - 1) + f.X(ijn - 1));
+    double cxp = 0.5 * (f.X(ij) + f.X(ijn));
+    double cym = 0.5 * (f.Y(ij) + f.Y(ijs));
+    double cyp = 0.5 * (f.Y(ij) + f.Y(ijn));
+    Coef c;
+    c.aw = fmax(fabs(cxm), L.du_x) + cxm;
+    c.ae = fmax(fabs(cxp), L.du_x) - cxp;
+    c.as = fmax(fabs(cym), L.du_y) + cym;
+    c.an = fmax(fabs(cyp), L.du_y) - cyp;
+    c.ap = ((c.aw + c.ae) + c.as) + c.an;
+    if (i == nx - 2) { c.ae += L.du_y; c.ap += L.du_y; }   /* V6:1010 uses areay/dy */
+    if (i == 1) { c.aw += L.du_y; c.ap += L.du_y; }
+    return c;
+}
+
+/* coefsc at one cell including the obstacle diffusion adds that the
+ * reference applies on the host in obstacle-list order (V6:872-963) */
+template <class FX>
+__device__ __forceinline__ Coef coefsc_at(const DevLevel &L, const FX &f, int i, int j,
+                                          uint8_t fl)
+{
+    const int nx = L.npx;
+    const int ij = j * nx + i, ijs = ij - nx;
+    double cxm = f.X(ij - 1), cxp = f.X(ij), cym = f.Y(ijs), cyp = f.Y(ij);
+    Coef c;
+    c.aw = fmax(fabs(cxm), L.ds_x) + cxm;
+    c.ae = fmax(fabs(cxp), L.ds_x) - cxp;
+    c.as = fmax(fabs(cym), L.ds_y) + cym;
+    c.an = fmax(fabs(cyp), L.ds_y) - cyp;
+    c.ap = ((c.aw + c.ae) + c.as) + c.an;
+    if (fl & MGF_F_SCADD) {
+        uint64_t code = L.sccode[ij];
+        int n_ae = (int)(code & 31u), n_aw = (int)((code >> 5) & 31u);
+        int n_an = (int)((code >> 10) & 31u), n_as = (int)((code >> 15) & 31u);
+        int n_ap = (int)((code >> 20) & 63u);
+        for (int k = 0; k < n_ae; ++k) c.ae += L.ds_x;
+        for (int k = 0; k < n_aw; ++k) c.aw += L.ds_x;
+        for (int k = 0; k < n_an; ++k) c.an += L.ds_y;
+        for (int k = 0; k < n_as; ++k) c.as += L.ds_y;
+        uint64_t seq = code >> 26;
+        for (int k = 0; k < n_ap; ++k) c.ap += ((seq >> k) & 1u) ? L.ds_y : L.ds_x;
+    }
+    return c;
+}
+
+#define RANGE2D(i0, i1, j0, j1)                                   \
+    const int i = (i0) + (int)(blockIdx.x * BX + threadIdx.x);    \
+    const int j = (j0) + (int)(blockIdx.y * BY + threadIdx.y);    \
+    if (i > (i1) || j > (j1)) return;
+
+static inline dim3 grid2d(int ni, int nj)
+{
+    return dim3((unsigned)((ni + BX - 1) / BX), (unsigned)((nj + BY - 1) / BY));
+}
+
+/* ---------------------------------------------------------------------- */
+/* fluxes, V6:678-694: j in [0,npy-2], i in [0,npx-2]                       */
+/* ---------------------------------------------------------------------- */
+__global__ void __launch_bounds__(BX * BY) k_fluxes(DevLevel L, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(0, L.npx - 2, 0, L.npy - 2);
+    const int ij = j * L.npx + i;
+    L.cx[ij] = (0.5 * L.u[ij]) * L.ax;
+    L.cy[ij] = (0.5 * L.v[ij]) * L.ay;
+}
+
+/* ---------------------------------------------------------------------- */
+/* coefu + walls + umom, V6:741-767, 815-860: j in [1,npy-2], i in [1,npx-3] */
+/* ---------------------------------------------------------------------- */
+__global__ void __launch_bounds__(BX * BY) k_asm_u(DevLevel L, DevPhys P, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(1, L.npx - 3, 1, L.npy - 2);
+    const int ij = j * L.npx + i;
+    Coef c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
+    double su = L.ax * (L.p[ij] - L.p[ij + 1]);
+    double ap = c.ap / P.relxm;
+    su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
+    ap = ap + ((L.flags[ij] & MGF_F_SPU) ? 1.0e30 : 0.0);
+    L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
+    L.ap[ij] = ap;
+    L.su[ij] = su;
+    L.apu[ij] = ap;
+}
+
+/* coefv + side walls + vmom, V6:769-793, 966-1016: j in [1,npy-3], i in [1,npx-2] */
+__global__ void __launch_bounds__(BX * BY) k_asm_v(DevLevel L, DevPhys P, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(1, L.npx - 2, 1, L.npy - 3);
+    const int ij = j * L.npx + i;
+    Coef c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
+    double su = L.ay * (L.p[ij] - L.p[ij + L.npx]);
+    double ap = c.ap / P.relxm;
+    su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
+    ap = ap + ((L.flags[ij] & MGF_F_SPV) ? 1.0e30 : 0.0);
+    L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
+    L.ap[ij] = ap;
+    L.su[ij] = su;
+    L.apv[ij] = ap;
+}
+
+/* coefsc + obstacle adds + scalar, V6:795-812, 861-964: [1,npy-2]x[1,npx-2] */
+__global__ void __launch_bounds__(BX * BY) k_asm_sc(DevLevel L, DevPhys P, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(1, L.npx - 2, 1, L.npy - 2);
+    const int ij = j * L.npx + i;
+    const uint8_t fl = L.flags[ij];
+    Coef c = coefsc_at(L, FluxArr{L.cx, L.cy}, i, j, fl);
+    const double spsc = (fl & MGF_F_SPSC) ? 1.0e30 : 0.0;
+    L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
+    L.su[ij] = spsc * P.sc_obs + L.ressc[ij];
+    L.ap[ij] = c.ap + spsc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* masserror + coefp (+ p' Dirichlet rows), V6:1018-1125                    */
+/* also the sum of |div| with a deterministic two-level tree               */
+/* ---------------------------------------------------------------------- */
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((tid & 63) == 0) lds[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (tid == 0) t = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(BX * BY) k_asm_p(DevLevel L, DevPhys P, DevCtl *ctl, int lev,
+                                                   int record, double *hist)
+{
+    if (ctl_done(ctl)) return;
+    __shared__ double lds[BY];
+    __shared__ int am_last;
+    const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int nx = L.npx;
+    double mag = 0.0;
+    if (i <= nx - 2 && j <= L.npy - 2) {
+        const int ij = j * nx + i, ijs = ij - nx;
+        double div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
+        double su = -div;
+        mag = fabs(div);
+        double aw = L.axax / L.apu[ij - 1];
+        double ae = L.axax / L.apu[ij];
+        double as = L.ayay / L.apv[ijs];
+        double an = L.ayay / L.apv[ij];
+        double ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
+        const double diag = 1.0e30;
+        if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
+        if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
+        if (P.ibc[2] == 2 && j == 1) { ap = ap + diag; su = 0.0; }
+        if (P.ibc[3] == 2 && j == L.npy - 2) { ap = ap + diag; su = 0.0; }
+        L.su[ij] = su;
+        L.pp[ij] = 0.0;
+        L.aw[ij] = aw; L.ae[ij] = ae; L.as[ij] = as; L.an[ij] = an;
+        L.ap[ij] = ap;
+    }
+    double bs = block_sum(mag, lds);
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const unsigned nblk = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+    if (tid == 0) {
+        L.partial[b] = bs;
+        __threadfence();
+        unsigned prev = atomicAdd(&ctl->counter[lev], 1u);
+        am_last = (prev == nblk - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    const volatile double *part = L.partial;
+    double t = 0.0;
+    for (unsigned k = (unsigned)tid; k < nblk; k += BX * BY) t += part[k];
+    double tot = block_sum(t, lds);
+    if (tid == 0) {
+        ctl->err[lev] = tot;
+        if (record) hist[ctl->cyc] = tot;
+        ctl->counter[lev] = 0u;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* smoothers.  (nx, ny) are the reference's solver arguments: points        */
+/* j in [1,ny-1], i in [1,nx-1] (V6:1265-1266, 1314-1315)                   */
+/* ---------------------------------------------------------------------- */
+
+/* one Jacobi sweep src -> dst; neighbours outside the swept range are read
+ * from `base` (the field itself), which is what the reference's full copy
+ * of phi into phi_new provides (V6:1254).  wj = 1 (V6:1241). */
+__global__ void __launch_bounds__(BX * BY) k_jacobi(DevLevel L, const double *src, double *dst,
+                                                    const double *base, int nx, int ny,
+                                                    const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(1, nx - 1, 1, ny - 1);
+    const int npx = L.npx;
+    const int ij = j * npx + i;
+    const double W = (i - 1 >= 1) ? src[ij - 1] : base[ij - 1];
+    const double E = (i + 1 <= nx - 1) ? src[ij + 1] : base[ij + 1];
+    const double S = (j - 1 >= 1) ? src[ij - npx] : base[ij - npx];
+    const double N = (j + 1 <= ny - 1) ? src[ij + npx] : base[ij + npx];
+    const double sum_nb = ((L.aw[ij] * W + L.ae[ij] * E) + L.as[ij] * S) + L.an[ij] * N;
+    const double g = (sum_nb + L.su[ij]) / (L.ap[ij] + 1.0e-30);
+    const double wj = 1.0;
+    dst[ij] = (1.0 - wj) * src[ij] + wj * g;
+}
+
+__global__ void __launch_bounds__(BX * BY) k_copy_range(double *dst, const double *src, int npx,
+                                                        int nx, int ny, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(1, nx - 1, 1, ny - 1);
+    dst[j * npx + i] = src[j * npx + i];
+}
+
+/* one colour of the red-black sweep, in place (V6:1308-1349) */
+__global__ void __launch_bounds__(BX * BY) k_rb(DevLevel L, double *phi, int nx, int ny,
+                                                int colour, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    if (j > ny - 1) return;
+    const int i0 = (((1 + j) & 1) == colour) ? 1 : 2;
+    const int i = i0 + 2 * (int)(blockIdx.x * BX + threadIdx.x);
+    if (i > nx - 1) return;
+    const int npx = L.npx;
+    const int ij = j * npx + i;
+    const double sum_nb = ((L.aw[ij] * phi[ij - 1] + L.ae[ij] * phi[ij + 1]) +
+                           L.as[ij] * phi[ij - npx]) + L.an[ij] * phi[ij + npx];
+    phi[ij] = (sum_nb + L.su[ij]) / L.ap[ij];
+}
+
+/* ---------------------------------------------------------------------- */
+/* update, V6:1127-1178                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* freshly updated p of every copy origin, taken before the update overwrites
+ * anything (the reference copies after its update loop, in list order) */
+__global__ void k_pstash(DevLevel L, DevPhys P, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= L.npcopy) return;
+    const int o = L.porig[s];
+    const int oi = o % L.npx, oj = o / L.npx;
+    const bool in = (oi >= 1 && oi <= L.npx - 2 && oj >= 1 && oj <= L.npy - 2);
+    L.pstash[s] = in ? L.p[o] + P.relxp * (L.pp[o] - 0.0) : L.p[o];
+}
+
+__global__ void __launch_bounds__(BX * BY) k_update(DevLevel L, DevPhys P, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    RANGE2D(1, L.npx - 2, 1, L.npy - 2);
+    const int nx = L.npx;
+    const int ij = j * nx + i;
+    const double pp = L.pp[ij];
+    L.u[ij] = L.u[ij] + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
+    L.v[ij] = L.v[ij] + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
+    if (L.flags[ij] & MGF_F_PCOPY)
+        L.p[ij] = L.pstash[L.pslot[ij]];
+    else
+        L.p[ij] = L.p[ij] + P.relxp * (pp - 0.0);
+}
+
+/* ---------------------------------------------------------------------- */
+/* periodic / outflow boundaries, V6:1179-1228                              */
+/* ---------------------------------------------------------------------- */
+__global__ void k_pbound_x(DevLevel L, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j > L.npy - 2) return;
+    const int ij = j * L.npx, ij1 = ij + L.npx - 3;
+    L.u[ij] = L.u[ij1];
+    L.u[ij1 + 1] = L.u[ij + 1];
+    L.v[ij] = L.v[ij1];
+    L.v[ij1 + 1] = L.v[ij + 1];
+}
+
+/* the y-periodic branch indexes ij + ny - 3 (V6:1203), so its iterations
+ * depend on each other; one thread runs them in the reference's order */
+__global__ void k_pbound_y(DevLevel L, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int nx = L.npx, ny = L.npy;
+    for (int i = 1; i < nx - 1; ++i) {
+        const int ij = i, ij1 = ij + ny - 3;
+        L.u[ij] = L.u[ij1];
+        L.u[ij1 + nx] = L.u[ij + nx];
+        L.v[ij] = L.v[ij1];
+        L.v[ij1 + nx] = L.v[ij + nx];
+    }
+}
+
+__global__ void k_outflow(DevLevel L, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j > L.npy - 1) return;
+    const int ij = j * L.npx + L.npx - 1;
+    L.u[ij - 1] = L.u[ij - 2];
+    L.v[ij] = L.v[ij - 1];
+    L.p[ij] = L.p[ij - 1];
+    L.sc[ij] = L.sc[ij - 1];
+}
+
+/* ---------------------------------------------------------------------- */
+/* restriction                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* restv, V6:1354-1389: coarse [1,nyc-2]x[1,nxc-2] */
+__global__ void __launch_bounds__(BX * BY) k_restv(DevLevel F, DevLevel C, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    if (ic > C.npx - 2 || jc > C.npy - 2) return;
+    const int nxf = F.npx;
+    const int ijc = jc * C.npx + ic;
+    const int ij = (2 * jc) * nxf + 2 * ic, ijw = ij - 1, ijs = ij - nxf;
+    C.u[ijc] = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0e-30);
+    C.v[ijc] = (F.v[ij] * F.ay + F.v[ijw] * F.ay) / (C.ay + 1.0e-30);
+    C.p[ijc] = 0.25 * (((F.p[ij] + F.p[ijw]) + F.p[ijs]) + F.p[ijs - 1]);
+    C.sc[ijc] = 0.25 * (((F.sc[ij] + F.sc[ijw]) + F.sc[ijs]) + F.sc[ijs - 1]);
+}
+
+/* momentum residual at a u face with un-relaxed, un-penalised coefficients
+ * (restru, V6:1496-1510 fine with resu, 1532-1545 coarse without) */
+__device__ __forceinline__ double rs_u(const DevLevel &L, int i, int j, double body, bool fine)
+{
+    const int nx = L.npx, ij = j * nx + i;
+    Coef c = coefu_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
+    const double *u = L.u;
+    double r = ((((c.aw * u[ij - 1] + c.ae * u[ij + 1]) + c.as * u[ij - nx]) + c.an * u[ij + nx]) -
+                c.ap * u[ij]) + L.ax * (L.p[ij] - L.p[ij + 1]);
+    if (fine) r = r + L.resu[ij];
+    r = r + body;
+    if (fine && L.ax == 0.0) r = 0.0;
+    return r;
+}
+
+__device__ __forceinline__ double rs_v(const DevLevel &L, int i, int j, double body, bool fine)
+{
+    const int nx = L.npx, ij = j * nx + i;
+    Coef c = coefv_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
+    const double *v = L.v;
+    double r = ((((c.aw * v[ij - 1] + c.ae * v[ij + 1]) + c.as * v[ij - nx]) + c.an * v[ij + nx]) -
+                c.ap * v[ij]) + L.ay * (L.p[ij] - L.p[ij + nx]);
+    if (fine) r = r + L.resv[ij];
+    r = r + body;
+    if (fine && L.ay == 0.0) r = 0.0;
+    return r;
+}
+
+__device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fine)
+{
+    const int nx = L.npx, ij = j * nx + i;
+    Coef c = coefsc_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j, L.flags[ij]);
+    const double *s = L.sc;
+    double r = (((c.aw * s[ij - 1] + c.ae * s[ij + 1]) + c.as * s[ij - nx]) + c.an * s[ij + nx]) -
+               c.ap * s[ij];
+    if (fine) r = r + L.ressc[ij];
+    return r;
+}
+
+/* FAS tau terms resu/resv/ressc on the coarse level (V6:1547-1563,
+ * 1632-1647, 1723-1739); the fine and coarse residuals are evaluated where
+ * they are needed instead of through the rs scratch array */
+__global__ void __launch_bounds__(BX * BY) k_tau(DevLevel F, DevLevel C, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int nxc = C.npx, nyc = C.npy;
+    if (ic > nxc - 2 || jc > nyc - 2) return;
+    const int ijc = jc * nxc + ic;
+    const int i = 2 * ic, j = 2 * jc;
+    if (ic <= nxc - 3) {
+        double a = rs_u(F, i, j, F.bodyx, true), b = rs_u(F, i, j - 1, F.bodyx, true);
+        double w0 = rs_u(F, i - 1, j, F.bodyx, true), w1 = rs_u(F, i - 1, j - 1, F.bodyx, true);
+        double e0 = rs_u(F, i + 1, j, F.bodyx, true), e1 = rs_u(F, i + 1, j - 1, F.bodyx, true);
+        double rc = rs_u(C, ic, jc, C.bodyx, false);
+        C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rc;
+    }
+    if (jc <= nyc - 3) {
+        double a = rs_v(F, i, j, F.bodyy, true), b = rs_v(F, i - 1, j, F.bodyy, true);
+        double s0 = rs_v(F, i, j - 1, F.bodyy, true), s1 = rs_v(F, i - 1, j - 1, F.bodyy, true);
+        double n0 = rs_v(F, i, j + 1, F.bodyy, true), n1 = rs_v(F, i - 1, j + 1, F.bodyy, true);
+        double rc = rs_v(C, ic, jc, C.bodyy, false);
+        C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rc;
+    }
+    {
+        double a = rs_sc(F, i, j, true), b = rs_sc(F, i, j - 1, true);
+        double c2 = rs_sc(F, i - 1, j, true), d = rs_sc(F, i - 1, j - 1, true);
+        double rc = rs_sc(C, ic, jc, false);
+        C.ressc[ijc] = (((a + b) + c2) + d) - rc;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* prolongation, V6:1745-2028, in gather form                               */
+/* ---------------------------------------------------------------------- */
+
+/* coarse corrections c = x_c - R x_f for u, v, p, sc (prolu 1762-1779,
+ * prolv 1838-1855, prolp 1911-1928, prolsc 1979-1995).  All four read only
+ * pre-prolongation fine data, so one pass serves them all; the entries the
+ * reference leaves at zero are never written here (zero since creation). */
+__global__ void __launch_bounds__(BX * BY) k_prol_c(DevLevel F, DevLevel C, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int nxc = C.npx, nyc = C.npy, nxf = F.npx;
+    if (ic > nxc - 2 || jc > nyc - 2) return;
+    const int ijc = jc * nxc + ic;
+    const int ij = (2 * jc) * nxf + 2 * ic, ijs = ij - nxf;
+    if (ic <= nxc - 3) {
+        double Ru = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0E-30);
+        C.cu[ijc] = C.u[ijc] - Ru;
+    }
+    if (jc <= nyc - 3) {
+        double Rv = (F.v[ij] * F.ay + F.v[ij - 1] * F.ay) / (C.ay + 1.0E-30);
+        C.cv[ijc] = C.v[ijc] - Rv;
+    }
+    {
+        double Rp = 0.25 * (((F.p[ij] + F.p[ij - 1]) + F.p[ijs]) + F.p[ijs - 1]);
+        double cp = C.p[ijc] - Rp;
+        if (F.flags[ij] & MGF_F_SPU) cp = 0.0;
+        C.cp[ijc] = cp;
+        double Rs = 0.25 * (((F.sc[ij] + F.sc[ij - 1]) + F.sc[ijs]) + F.sc[ijs - 1]);
+        C.csc[ijc] = C.sc[ijc] - Rs;
+    }
+}
+
+/* face correction of prolu/prolv for coarse face k along the face direction,
+ * mixing in the neighbour across it (3/4, 1/4), zero where the fine face at
+ * (2ic, 2jc) is penalised (V6:1794-1798, 1870-1874) */
+__device__ __forceinline__ double corr_u(const DevLevel &F, const DevLevel &C, int ic, int jc,
+                                         bool upper)
+{
+    const int nxc = C.npx;
+    const int ijc = jc * nxc + ic;
+    const double f1 = 3. / 4., f2 = 1. / 4.;
+    double c = upper ? f1 * C.cu[ijc] + f2 * C.cu[ijc + nxc] : f1 * C.cu[ijc] + f2 * C.cu[ijc - nxc];
+    if (F.flags[(2 * jc) * F.npx + 2 * ic] & MGF_F_SPU) c = 0.0;
+    return c;
+}
+
+__device__ __forceinline__ double corr_v(const DevLevel &F, const DevLevel &C, int ic, int jc,
+                                         bool right)
+{
+    const int nxc = C.npx;
+    const int ijc = jc * nxc + ic;
+    const double f1 = 3. / 4., f2 = 1. / 4.;
+    double c = right ? f1 * C.cv[ijc] + f2 * C.cv[ijc + 1] : f1 * C.cv[ijc] + f2 * C.cv[ijc - 1];
+    if (F.flags[(2 * jc) * F.npx + 2 * ic] & MGF_F_SPV) c = 0.0;
+    return c;
+}
+
+/* bilinear 9/16, 3/16, 3/16, 1/16 cell correction (V6:1945-1955) */
+__device__ __forceinline__ double corr_cell(const double *c, int nxc, int ic, int jc, int di, int dj)
+{
+    const double f1 = 9. / 16., f2 = 3. / 16., f3 = 3. / 16., f4 = 1. / 16.;
+    const int ijc = jc * nxc + ic;
+    return ((f1 * c[ijc] + f2 * c[ijc + di]) + f3 * c[ijc + dj * nxc]) + f4 * c[ijc + di + dj * nxc];
+}
+
+/* every fine point gathers its corrections; odd faces, which the reference
+ * updates twice (atomics in its GPU build), take the two halves in its CPU
+ * loop order (smaller coarse index first) */
+__global__ void __launch_bounds__(BX * BY) k_prol_apply(DevLevel F, DevLevel C, const DevCtl *ctl)
+{
+    if (ctl_done(ctl)) return;
+    const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int nxf = F.npx, nyf = F.npy, nxc = C.npx, nyc = C.npy;
+    if (i > nxf - 2 || j > nyf - 2) return;
+    const int ij = j * nxf + i;
+    /* u faces: i in [1, nxf-3], coarse faces ic in [1, nxc-3] */
+    if (i <= nxf - 3) {
+        const int jc = (j + 1) >> 1;
+        const bool upper = !(j & 1);              /* row 2jc: corr1; row 2jc-1: corr2 */
+        double u = F.u[ij];
+        if (!(i & 1)) {
+            u = u + corr_u(F, C, i >> 1, jc, upper);
+        } else {
+            const int icl = (i - 1) >> 1, icr = (i + 1) >> 1;
+            if (icl >= 1) u = u + 0.5 * corr_u(F, C, icl, jc, upper);
+            if (icr <= nxc - 3) u = u + 0.5 * corr_u(F, C, icr, jc, upper);
+        }
+        F.u[ij] = u;
+    }
+    /* v faces: j in [1, nyf-3], coarse faces jc in [1, nyc-3] */
+    if (j <= nyf - 3) {
+        const int ic = (i + 1) >> 1;
+        const bool right = !(i & 1);              /* column 2ic: corr1; 2ic-1: corr2 */
+        double v = F.v[ij];
+        if (!(j & 1)) {
+            v = v + corr_v(F, C, ic, j >> 1, right);
+        } else {
+            const int jcl = (j - 1) >> 1, jcu = (j + 1) >> 1;
+            if (jcl >= 1) v = v + 0.5 * corr_v(F, C, ic, jcl, right);
+            if (jcu <= nyc - 3) v = v + 0.5 * corr_v(F, C, ic, jcu, right);
+        }
+        F.v[ij] = v;
+    }
+    /* cells */
+    {
+        const int ic = (i + 1) >> 1, jc = (j + 1) >> 1;
+        const int di = (i & 1) ? -1 : 1, dj = (j & 1) ? -1 : 1;
+        F.p[ij] = F.p[ij] + corr_cell(C.cp, nxc, ic, jc, di, dj);
+        F.sc[ij] = F.sc[ij] + corr_cell(C.csc, nxc, ic, jc, di, dj);
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* end of a V-cycle: stopping test of V6:247-248                            */
+/* ---------------------------------------------------------------------- */
+__global__ void k_end_cycle(DevCtl *ctl, const double *hist)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ctl->done) return;
+    const double h = hist[ctl->cyc];
+    if (!(h == h) || h == __builtin_huge_val() || h == -__builtin_huge_val()) ctl->nonfinite = 1;
+    if (h < ctl->tol) ctl->done = 1;
+    ctl->cyc = ctl->cyc + 1;
+}
+
+__global__ void k_fill(double *a, double v, size_t n)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
+/* ====================================================================== */
+/* launch wrappers                                                         */
+/* ====================================================================== */
+
+int mgf_partial_blocks(int npx, int npy)
+{
+    dim3 g = grid2d(npx - 2, npy - 2);
+    return (int)(g.x * g.y);
+}
+
+void mgf_launch_fluxes(const DevLevel &L, const DevCtl *ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fluxes, grid2d(L.npx - 1, L.npy - 1), dim3(BX, BY), 0, st, L, ctl);
+}
+
+void mgf_launch_asm_u(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+{
+    if (L.npx - 3 < 1) return;
+    hipLaunchKernelGGL(k_asm_u, grid2d(L.npx - 3, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl);
+}
+
+void mgf_launch_asm_v(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+{
+    if (L.npy - 3 < 1) return;
+    hipLaunchKernelGGL(k_asm_v, grid2d(L.npx - 2, L.npy - 3), dim3(BX, BY), 0, st, L, P, ctl);
+}
+
+void mgf_launch_asm_p(const DevLevel &L, const DevPhys &P, DevCtl *ctl, int lev, int record,
+                      double *hist, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_asm_p, grid2d(L.npx - 2, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl,
+                       lev, record, hist);
+}
+
+void mgf_launch_asm_sc(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_asm_sc, grid2d(L.npx - 2, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl);
+}
+
+void mgf_launch_smooth(const DevLevel &L, int field, int isolve, int nsw, const DevCtl *ctl,
+                       hipStream_t st)
+{
+    double *phi;
+    int nx, ny;
+    switch (field) {
+    case 0: phi = L.u; nx = L.npx - 2; ny = L.npy - 1; break;
+    case 1: phi = L.v; nx = L.npx - 1; ny = L.npy - 2; break;
+    case 2: phi = L.pp; nx = L.npx - 1; ny = L.npy - 1; break;
+    default: phi = L.sc; nx = L.npx - 1; ny = L.npy - 1; break;
+    }
+    if (nx - 1 < 1 || ny - 1 < 1 || nsw <= 0) return;
+    if (isolve == 1) {
+        const double *cur = phi;
+        double *nxt = L.tmp;
+        for (int it = 0; it < nsw; ++it) {
+            hipLaunchKernelGGL(k_jacobi, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, L, cur, nxt,
+                               (const double *)phi, nx, ny, ctl);
+            double *t = (double *)cur;
+            cur = nxt;
+            nxt = t;
+        }
+        if (cur != phi)
+            hipLaunchKernelGGL(k_copy_range, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, phi,
+                               cur, L.npx, nx, ny, ctl);
+    } else if (isolve == 2) {
+        dim3 g((unsigned)(((nx - 1 + 1) / 2 + BX - 1) / BX), (unsigned)((ny - 1 + BY - 1) / BY));
+        for (int it = 0; it < nsw; ++it) {
+            hipLaunchKernelGGL(k_rb, g, dim3(BX, BY), 0, st, L, phi, nx, ny, 0, ctl);
+            hipLaunchKernelGGL(k_rb, g, dim3(BX, BY), 0, st, L, phi, nx, ny, 1, ctl);
+        }
+    }
+}
+
+void mgf_launch_update(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+{
+    if (L.npcopy > 0)
+        hipLaunchKernelGGL(k_pstash, dim3((unsigned)((L.npcopy + 255) / 256)), dim3(256), 0, st, L,
+                           P, ctl);
+    hipLaunchKernelGGL(k_update, grid2d(L.npx - 2, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl);
+}
+
+void mgf_launch_pbound(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+{
+    if (P.ibc[0] == 3)
+        hipLaunchKernelGGL(k_pbound_x, dim3((unsigned)((L.npy + 255) / 256)), dim3(256), 0, st, L,
+                           ctl);
+    if (P.ibc[2] == 3) hipLaunchKernelGGL(k_pbound_y, dim3(1), dim3(64), 0, st, L, ctl);
+}
+
+void mgf_launch_outflow(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+{
+    if (P.ibc[1] == 4)
+        hipLaunchKernelGGL(k_outflow, dim3((unsigned)((L.npy + 255) / 256)), dim3(256), 0, st, L,
+                           ctl);
+}
+
+void mgf_launch_restrict(const DevLevel &F, const DevLevel &C, const DevPhys &P,
+                         const DevCtl *ctl, hipStream_t st)
+{
+    (void)P;
+    dim3 g = grid2d(C.npx - 2, C.npy - 2);
+    hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, F, C, ctl);
+    hipLaunchKernelGGL(k_tau, g, dim3(BX, BY), 0, st, F, C, ctl);
+}
+
+void mgf_launch_prolong(const DevLevel &F, const DevLevel &C, const DevCtl *ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_prol_c, grid2d(C.npx - 2, C.npy - 2), dim3(BX, BY), 0, st, F, C, ctl);
+    hipLaunchKernelGGL(k_prol_apply, grid2d(F.npx - 2, F.npy - 2), dim3(BX, BY), 0, st, F, C, ctl);
+}
+
+void mgf_launch_end_cycle(DevCtl *ctl, const double *hist, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_end_cycle, dim3(1), dim3(64), 0, st, ctl, hist);
+}
+
+void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, v, n);
+}

@@ -1,0 +1,16 @@
+#ifndef MGF_OUTPUT_H
+#define MGF_OUTPUT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int mgf_write_vtk(const char *fname, int npx, int npy, double dx, double dy, const double *u,
+                  const double *v, const double *p, const double *sc, const unsigned char *obs,
+                  int iter, int print_stats);
+int mgf_write_convergence(const char *fname, const double *hist, int nrows, int nvcycle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,12 @@
+"""MI355X-native SIMPLE / FAS-multigrid solver (Python side).
+
+The solver itself is native: host set-up in C and HIP kernels for gfx950 in
+libmgfvm.so, reached through the C ABI of include/mgfvm.h.  This package
+holds the ctypes binding (`Solver`) and the input-file tools (`inputs`).
+"""
+from . import inputs
+from ._lib import (EXPORTS, KERNEL_SWEEP_PP, KERNEL_SWEEP_U, LIB_PATH, PREC_FP64,
+                   PREC_MIXED, MgfError, Solver, lib)
+
+__all__ = ["inputs", "Solver", "MgfError", "lib", "LIB_PATH", "EXPORTS", "PREC_FP64",
+           "PREC_MIXED", "KERNEL_SWEEP_PP", "KERNEL_SWEEP_U"]

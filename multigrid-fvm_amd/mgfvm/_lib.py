@@ -1,0 +1,216 @@
+"""ctypes binding of libmgfvm.so (the C ABI in include/mgfvm.h).
+
+This is a thin host-side mirror for Python callers: every call goes straight
+to the native library, which runs the HIP kernels.  There is no fallback: if
+the library or a GPU is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libmgfvm.so")
+
+# declared entry points (kept in sync with include/mgfvm.h; tests check both)
+EXPORTS = [
+    "mgf_problem_read", "mgf_problem_free", "mgf_options_default", "mgf_error_string",
+    "mgf_abi_version", "mgf_create", "mgf_destroy", "mgf_last_error", "mgf_num_levels",
+    "mgf_level_dims", "mgf_stream", "mgf_relax", "mgf_restrict", "mgf_prolong",
+    "mgf_vcycle", "mgf_solve", "mgf_get_field", "mgf_set_field", "mgf_get_fields",
+    "mgf_time_kernel",
+]
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+KERNEL_SWEEP_PP = 0
+KERNEL_SWEEP_U = 1
+
+
+class MgfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mgfvm error {code}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("device", ctypes.c_int), ("strict_limits", ctypes.c_int)]
+
+
+class ProblemStruct(ctypes.Structure):
+    _fields_ = [
+        ("nfx", ctypes.c_int), ("nfy", ctypes.c_int), ("ngrid", ctypes.c_int),
+        ("ig_obs", ctypes.c_int), ("ibeg_obs", ctypes.c_int), ("ig_inlt", ctypes.c_int),
+        ("ig_outlt", ctypes.c_int), ("nvcycle", ctypes.c_int), ("nswpm", ctypes.c_int),
+        ("nswpp", ctypes.c_int), ("isolve", ctypes.c_int), ("omega", ctypes.c_double),
+        ("convergence", ctypes.c_double), ("xl", ctypes.c_double), ("yl", ctypes.c_double),
+        ("amuf", ctypes.c_double), ("prsc", ctypes.c_double), ("relxm", ctypes.c_double),
+        ("relxp", ctypes.c_double), ("ibc", ctypes.c_int * 4),
+        ("bc", (ctypes.c_double * 4) * 4), ("guess", ctypes.c_double * 4),
+        ("sc_obs", ctypes.c_double), ("nobs", ctypes.c_int),
+        ("obs", ctypes.POINTER(ctypes.c_int)), ("ninlt", ctypes.c_int),
+        ("noutlt", ctypes.c_int), ("inlt_j", ctypes.POINTER(ctypes.c_int)),
+        ("outlt_j", ctypes.POINTER(ctypes.c_int)),
+        ("inlt_v", ctypes.POINTER(ctypes.c_double)),
+        ("outlt_v", ctypes.POINTER(ctypes.c_double)),
+        ("dpdxm", ctypes.c_double), ("dpdym", ctypes.c_double),
+        ("inlet_u", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH}; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+        P_i, P_d = ctypes.POINTER(ip), ctypes.POINTER(dp)
+        L.mgf_problem_read.argtypes = [cp, cp, ctypes.POINTER(ctypes.POINTER(ProblemStruct))]
+        L.mgf_problem_free.argtypes = [ctypes.POINTER(ProblemStruct)]
+        L.mgf_options_default.argtypes = [ctypes.POINTER(Options)]
+        L.mgf_error_string.restype = cp
+        L.mgf_create.argtypes = [ctypes.POINTER(ProblemStruct), ctypes.POINTER(Options), ctypes.POINTER(vp)]
+        L.mgf_destroy.argtypes = [vp]
+        L.mgf_last_error.argtypes = [vp]
+        L.mgf_last_error.restype = cp
+        L.mgf_num_levels.argtypes = [vp]
+        L.mgf_level_dims.argtypes = [vp, ip, P_i, P_i]
+        L.mgf_stream.argtypes = [vp]
+        L.mgf_stream.restype = vp
+        L.mgf_relax.argtypes = [vp, ip, ip, P_d]
+        L.mgf_restrict.argtypes = [vp, ip]
+        L.mgf_prolong.argtypes = [vp, ip]
+        L.mgf_vcycle.argtypes = [vp, ip, P_d]
+        L.mgf_solve.argtypes = [vp, ip, dp, P_d, P_i, P_i]
+        L.mgf_get_field.argtypes = [vp, ip, cp, P_d]
+        L.mgf_set_field.argtypes = [vp, ip, cp, P_d]
+        L.mgf_time_kernel.argtypes = [vp, ip, ip, ip, P_d, P_d]
+        _lib = L
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def read_problem(inputdata: str, pattern: str):
+    """Parse the file pair with the library's own reader; returns an owned
+    pointer to be released with free_problem()."""
+    L = lib()
+    p = ctypes.POINTER(ProblemStruct)()
+    rc = L.mgf_problem_read(inputdata.encode(), pattern.encode(), ctypes.byref(p))
+    if rc != 0:
+        raise MgfError(rc, L.mgf_error_string().decode())
+    return p
+
+
+def free_problem(p):
+    lib().mgf_problem_free(p)
+
+
+class Solver:
+    """One problem resident on one GPU."""
+
+    def __init__(self, inputdata: str, pattern: str, precision: int = PREC_FP64,
+                 use_graph: bool = True, device: int = -1, strict_limits: bool = False):
+        L = lib()
+        prob = read_problem(inputdata, pattern)
+        try:
+            self._init_from(prob, precision, use_graph, device, strict_limits)
+        finally:
+            free_problem(prob)
+
+    def _init_from(self, prob, precision, use_graph, device, strict_limits):
+        L = lib()
+        opt = Options()
+        L.mgf_options_default(ctypes.byref(opt))
+        opt.precision = precision
+        opt.use_graph = 1 if use_graph else 0
+        opt.device = device
+        opt.strict_limits = 1 if strict_limits else 0
+        h = ctypes.c_void_p()
+        rc = L.mgf_create(prob, ctypes.byref(opt), ctypes.byref(h))
+        if rc != 0:
+            raise MgfError(rc, L.mgf_error_string().decode())
+        self.h = h
+        self.ngrid = L.mgf_num_levels(h)
+        self.nvcycle = prob.contents.nvcycle
+        self.convergence = prob.contents.convergence
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise MgfError(rc, lib().mgf_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def dims(self, ig: int) -> Tuple[int, int]:
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().mgf_level_dims(self.h, ig, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def stream(self) -> int:
+        return lib().mgf_stream(self.h)
+
+    def get(self, ig: int, name: str) -> np.ndarray:
+        nx, ny = self.dims(ig)
+        out = np.empty(nx * ny, np.float64)
+        self._check(lib().mgf_get_field(self.h, ig, name.encode(), _dptr(out)))
+        return out.reshape(ny, nx)
+
+    def set(self, ig: int, name: str, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._check(lib().mgf_set_field(self.h, ig, name.encode(), _dptr(a)))
+
+    def relax(self, ig: int, leg: int) -> float:
+        e = ctypes.c_double()
+        self._check(lib().mgf_relax(self.h, ig, leg, ctypes.byref(e)))
+        return e.value
+
+    def restrict(self, ig: int):
+        self._check(lib().mgf_restrict(self.h, ig))
+
+    def prolong(self, igc: int):
+        self._check(lib().mgf_prolong(self.h, igc))
+
+    def vcycle(self, icyc: int) -> float:
+        e = ctypes.c_double()
+        self._check(lib().mgf_vcycle(self.h, icyc, ctypes.byref(e)))
+        return e.value
+
+    def solve(self, nvcycle: Optional[int] = None, tol: Optional[float] = None):
+        nv = self.nvcycle if nvcycle is None else nvcycle
+        tl = self.convergence if tol is None else tol
+        hist = np.zeros(max(nv, 1), np.float64)
+        nc, cv = ctypes.c_int(), ctypes.c_int()
+        rc = lib().mgf_solve(self.h, nv, tl, _dptr(hist), ctypes.byref(nc), ctypes.byref(cv))
+        if rc not in (0, -8):
+            self._check(rc)
+        return hist[: nc.value].copy(), nc.value, bool(cv.value)
+
+    def time_kernel(self, kernel: int, ig: int, reps: int):
+        us, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().mgf_time_kernel(self.h, kernel, ig, reps, ctypes.byref(us), ctypes.byref(b)))
+        return us.value, b.value

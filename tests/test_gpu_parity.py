@@ -1,0 +1,142 @@
+"""Parity of the HIP solver (through the C ABI) with the CPU restatement.
+
+FP64 bar: every state array (u, v, p, pp, sc, the FAS terms resu/resv/ressc
+and the momentum diagonals apu/apv) bit-identical on every level, after
+single steps, after whole V-cycles and after a graph-replayed solve.  The
+recorded residual sum(|div|) is summed in a different order on the GPU, so it
+is compared with a relative tolerance of 1e-12 (the bound for summing
+non-negative terms is n*eps ~ 2e-12 at 1024^2 points); the cycle count and
+the converged flag must be equal.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import orc
+
+pytestmark = pytest.mark.gpu
+
+STATE = ["u", "v", "p", "pp", "sc", "resu", "resv", "ressc", "apu", "apv"]
+HIST_RTOL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(tmp_path, name, **kw):
+    import mgfvm
+    pin, ppat = cases.write_case(name, str(tmp_path), **kw)
+    return mgfvm.Solver(pin, ppat), orc.Oracle(pin, ppat)
+
+
+def _diff_report(gpu, ora, levels, names=STATE):
+    bad = []
+    for ig in levels:
+        for n in names:
+            g, o = gpu.get(ig, n), ora.get(ig, n)
+            if not np.array_equal(g.view(np.uint64), o.view(np.uint64)):
+                d = np.abs(g - o)
+                idx = np.unravel_index(np.nanargmax(d), d.shape) if np.isfinite(d).any() else None
+                bad.append(f"L{ig}.{n}: max|d|={np.nanmax(d):.3e} at {idx} "
+                           f"(gpu {g[idx] if idx else 'nan'}, ref {o[idx] if idx else 'nan'}), "
+                           f"{int((g.view(np.uint64) != o.view(np.uint64)).sum())} differ")
+    return bad
+
+
+@pytest.mark.parametrize("name", cases.SMALL + ["cfg0"])
+def test_initial_state(tmp_path, name):
+    _say(f"[gpu] initial state {name}")
+    gpu, ora = _pair(tmp_path, name)
+    bad = _diff_report(gpu, ora, range(ora.ngrid), ["u", "v", "p", "sc", "apu", "apv", "pp"])
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("name", cases.SMALL + ["cfg0_rb"])
+def test_step_by_step_vcycle(tmp_path, name):
+    """Walk two V-cycles step by step (relax / restrict / prolong) and compare
+    after every step, so a mismatch names the step that caused it."""
+    _say(f"[gpu] steps {name}")
+    gpu, ora = _pair(tmp_path, name)
+    ng = ora.ngrid
+    for cyc in range(2):
+        for ig in range(ng):
+            eg, eo = gpu.relax(ig, 0), ora.relax(ig, 0)
+            bad = _diff_report(gpu, ora, [ig])
+            assert not bad, f"cycle {cyc} relax down L{ig}:\n" + "\n".join(bad)
+            assert eg == pytest.approx(eo, rel=HIST_RTOL, abs=0.0)
+            if ig != ng - 1:
+                gpu.restrict(ig)
+                ora.restrict(ig)
+                bad = _diff_report(gpu, ora, [ig, ig + 1])
+                assert not bad, f"cycle {cyc} restrict L{ig}:\n" + "\n".join(bad)
+        for ig in range(ng - 1, 0, -1):
+            gpu.prolong(ig)
+            ora.prolong(ig)
+            bad = _diff_report(gpu, ora, [ig - 1, ig])
+            assert not bad, f"cycle {cyc} prolong L{ig}:\n" + "\n".join(bad)
+            if ig != 1:
+                gpu.relax(ig - 1, 1)
+                ora.relax(ig - 1, 1)
+                bad = _diff_report(gpu, ora, [ig - 1])
+                assert not bad, f"cycle {cyc} relax up L{ig-1}:\n" + "\n".join(bad)
+
+
+@pytest.mark.parametrize("name", ["small_jac", "small_rb", "inlets", "ibc3", "rand64"])
+def test_vcycles(tmp_path, name):
+    _say(f"[gpu] vcycles {name}")
+    gpu, ora = _pair(tmp_path, name)
+    for icyc in range(5):
+        eg, eo = gpu.vcycle(icyc), ora.vcycle(icyc)
+        assert eg == pytest.approx(eo, rel=HIST_RTOL, abs=0.0), f"cycle {icyc}"
+    bad = _diff_report(gpu, ora, range(ora.ngrid))
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("name", cases.SMALL + ["cfg0", "cfg0_rb", "stag128"])
+def test_solve_matches_oracle(tmp_path, name):
+    """Whole solve with the captured V-cycle graph and the device-side
+    stopping test: same cycle count, same converged flag, same fields."""
+    _say(f"[gpu] solve {name}")
+    gpu, ora = _pair(tmp_path, name)
+    p = cases.CASES[name]
+    hg, ng_, cg = gpu.solve(p.nvcycle, p.convergence)
+    ho, no, co = ora.solve(p.nvcycle, p.convergence)
+    assert ng_ == no and cg == co
+    np.testing.assert_allclose(hg, ho, rtol=HIST_RTOL, atol=0.0)
+    bad = _diff_report(gpu, ora, range(ora.ngrid))
+    assert not bad, "\n".join(bad)
+
+
+def test_solve_without_graph_equals_graph(tmp_path):
+    import mgfvm
+    pin, ppat = cases.write_case("cfg0_rb", str(tmp_path))
+    _say("[gpu] graph vs eager cfg0_rb")
+    a = mgfvm.Solver(pin, ppat, use_graph=True)
+    b = mgfvm.Solver(pin, ppat, use_graph=False)
+    ha, na, ca = a.solve()
+    hb, nb, cb = b.solve()
+    assert (na, ca) == (nb, cb)
+    assert np.array_equal(ha, hb)
+    for n in STATE:
+        assert np.array_equal(a.get(0, n), b.get(0, n)), n
+
+
+def test_solve_twice_restarts_counting(tmp_path):
+    """A second mgf_solve continues from the current state with a fresh
+    cycle counter; replaying the graph again must not corrupt anything."""
+    import mgfvm
+    pin, ppat = cases.write_case("small_rb", str(tmp_path))
+    _say("[gpu] solve twice small_rb")
+    gpu = mgfvm.Solver(pin, ppat)
+    ora = orc.Oracle(pin, ppat)
+    for _ in range(2):
+        hg, ng_, _ = gpu.solve(7, 0.0)
+        ho, no, _ = ora.solve(7, 0.0)
+        assert ng_ == no == 7
+        np.testing.assert_allclose(hg, ho, rtol=HIST_RTOL, atol=0.0)
+    bad = _diff_report(gpu, ora, range(ora.ngrid))
+    assert not bad, "\n".join(bad)
