@@ -19,6 +19,7 @@ struct DevLevel {
     int npx, npy;
     int npcopy;
     int has_obs;
+    int ext;              /* level below the obstacle level: coarsened obstacles */
     /* metrics, each computed on the host with the reference's expression */
     double ax, ay;        /* areax (= dy), areay (= dx) */
     double du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, wall adds */

@@ -411,7 +411,8 @@ __global__ void __launch_bounds__(BX * BY) k_restv(DevLevel F, DevLevel C, const
 
 /* momentum residual at a u face with un-relaxed, un-penalised coefficients
  * (restru, V6:1496-1510 fine with resu, 1532-1545 coarse without) */
-__device__ __forceinline__ double rs_u(const DevLevel &L, int i, int j, double body, bool fine)
+__device__ __forceinline__ double rs_u(const DevLevel &L, int i, int j, double body, bool fine,
+                                       bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefu_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
@@ -421,10 +422,12 @@ __device__ __forceinline__ double rs_u(const DevLevel &L, int i, int j, double b
     if (fine) r = r + L.resu[ij];
     r = r + body;
     if (fine && L.ax == 0.0) r = 0.0;
+    if (mask && (L.flags[ij] & MGF_F_SPU)) r = 0.0;
     return r;
 }
 
-__device__ __forceinline__ double rs_v(const DevLevel &L, int i, int j, double body, bool fine)
+__device__ __forceinline__ double rs_v(const DevLevel &L, int i, int j, double body, bool fine,
+                                       bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefv_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
@@ -434,10 +437,12 @@ __device__ __forceinline__ double rs_v(const DevLevel &L, int i, int j, double b
     if (fine) r = r + L.resv[ij];
     r = r + body;
     if (fine && L.ay == 0.0) r = 0.0;
+    if (mask && (L.flags[ij] & MGF_F_SPV)) r = 0.0;
     return r;
 }
 
-__device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fine)
+__device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fine,
+                                        bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefsc_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j, L.flags[ij]);
@@ -445,6 +450,7 @@ __device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fi
     double r = (((c.aw * s[ij - 1] + c.ae * s[ij + 1]) + c.as * s[ij - nx]) + c.an * s[ij + nx]) -
                c.ap * s[ij];
     if (fine) r = r + L.ressc[ij];
+    if (mask && (L.flags[ij] & MGF_F_SPSC)) r = 0.0;
     return r;
 }
 
@@ -460,23 +466,27 @@ __global__ void __launch_bounds__(BX * BY) k_tau(DevLevel F, DevLevel C, const D
     if (ic > nxc - 2 || jc > nyc - 2) return;
     const int ijc = jc * nxc + ic;
     const int i = 2 * ic, j = 2 * jc;
+    /* onto a level below the obstacle level the penalised fine residuals
+     * are dropped (their nonzero value is balanced by the 1e30 penalty,
+     * which the residual leaves out); never the case when ig_obs == ngrid */
+    const bool m = C.ext != 0;
     if (ic <= nxc - 3) {
-        double a = rs_u(F, i, j, F.bodyx, true), b = rs_u(F, i, j - 1, F.bodyx, true);
-        double w0 = rs_u(F, i - 1, j, F.bodyx, true), w1 = rs_u(F, i - 1, j - 1, F.bodyx, true);
-        double e0 = rs_u(F, i + 1, j, F.bodyx, true), e1 = rs_u(F, i + 1, j - 1, F.bodyx, true);
+        double a = rs_u(F, i, j, F.bodyx, true, m), b = rs_u(F, i, j - 1, F.bodyx, true, m);
+        double w0 = rs_u(F, i - 1, j, F.bodyx, true, m), w1 = rs_u(F, i - 1, j - 1, F.bodyx, true, m);
+        double e0 = rs_u(F, i + 1, j, F.bodyx, true, m), e1 = rs_u(F, i + 1, j - 1, F.bodyx, true, m);
         double rc = rs_u(C, ic, jc, C.bodyx, false);
         C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rc;
     }
     if (jc <= nyc - 3) {
-        double a = rs_v(F, i, j, F.bodyy, true), b = rs_v(F, i - 1, j, F.bodyy, true);
-        double s0 = rs_v(F, i, j - 1, F.bodyy, true), s1 = rs_v(F, i - 1, j - 1, F.bodyy, true);
-        double n0 = rs_v(F, i, j + 1, F.bodyy, true), n1 = rs_v(F, i - 1, j + 1, F.bodyy, true);
+        double a = rs_v(F, i, j, F.bodyy, true, m), b = rs_v(F, i - 1, j, F.bodyy, true, m);
+        double s0 = rs_v(F, i, j - 1, F.bodyy, true, m), s1 = rs_v(F, i - 1, j - 1, F.bodyy, true, m);
+        double n0 = rs_v(F, i, j + 1, F.bodyy, true, m), n1 = rs_v(F, i - 1, j + 1, F.bodyy, true, m);
         double rc = rs_v(C, ic, jc, C.bodyy, false);
         C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rc;
     }
     {
-        double a = rs_sc(F, i, j, true), b = rs_sc(F, i, j - 1, true);
-        double c2 = rs_sc(F, i - 1, j, true), d = rs_sc(F, i - 1, j - 1, true);
+        double a = rs_sc(F, i, j, true, m), b = rs_sc(F, i, j - 1, true, m);
+        double c2 = rs_sc(F, i - 1, j, true, m), d = rs_sc(F, i - 1, j - 1, true, m);
         double rc = rs_sc(C, ic, jc, false);
         C.ressc[ijc] = (((a + b) + c2) + d) - rc;
     }
@@ -499,21 +509,31 @@ __global__ void __launch_bounds__(BX * BY) k_prol_c(DevLevel F, DevLevel C, cons
     if (ic > nxc - 2 || jc > nyc - 2) return;
     const int ijc = jc * nxc + ic;
     const int ij = (2 * jc) * nxf + 2 * ic, ijs = ij - nxf;
+    /* corrections from penalised points of a level below the obstacle
+     * level carry no information about the finer level and are dropped */
+    const uint8_t cfl = C.ext ? C.flags[ijc] : (uint8_t)0;
     if (ic <= nxc - 3) {
         double Ru = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0E-30);
-        C.cu[ijc] = C.u[ijc] - Ru;
+        double cu = C.u[ijc] - Ru;
+        if (cfl & MGF_F_SPU) cu = 0.0;
+        C.cu[ijc] = cu;
     }
     if (jc <= nyc - 3) {
         double Rv = (F.v[ij] * F.ay + F.v[ij - 1] * F.ay) / (C.ay + 1.0E-30);
-        C.cv[ijc] = C.v[ijc] - Rv;
+        double cv = C.v[ijc] - Rv;
+        if (cfl & MGF_F_SPV) cv = 0.0;
+        C.cv[ijc] = cv;
     }
     {
         double Rp = 0.25 * (((F.p[ij] + F.p[ij - 1]) + F.p[ijs]) + F.p[ijs - 1]);
         double cp = C.p[ijc] - Rp;
         if (F.flags[ij] & MGF_F_SPU) cp = 0.0;
+        if (cfl & MGF_F_SPSC) cp = 0.0;
         C.cp[ijc] = cp;
         double Rs = 0.25 * (((F.sc[ij] + F.sc[ij - 1]) + F.sc[ijs]) + F.sc[ijs - 1]);
-        C.csc[ijc] = C.sc[ijc] - Rs;
+        double cs = C.sc[ijc] - Rs;
+        if (cfl & MGF_F_SPSC) cs = 0.0;
+        C.csc[ijc] = cs;
     }
 }
 

@@ -9,9 +9,12 @@
  *   coarse face areas and coarse inlet/outlet values: rest_geom,
  *       Version-6-26-04.c:1391-1473
  *
- * Coarse obstacle semantics: obstacles act on levels ig < ig_obs only (the
- * reference's init does the same, V6:618; its coefsc/update loops read
- * uninitialised index arrays on levels >= ig_obs, which is undefined).
+ * Levels ig >= ig_obs, on which the reference reads uninitialised obstacle
+ * index arrays (V6:471, 618, 939-963, 1153-1177): a cell there is an obstacle
+ * when all of its children are (mgf_plan_level.ext marks these levels; the
+ * restriction onto and prolongation from them mask penalised points, see
+ * the kernels).  With ig_obs == ngrid nothing here differs from the
+ * reference.
  */
 #include <math.h>
 #include <stdio.h>
@@ -87,7 +90,8 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
     pl->lv = (mgf_plan_level *)calloc((size_t)ng, sizeof(mgf_plan_level));
 
     /* per-level index ranges of obstacles / inlets / outlets (V6:468-502) */
-    int *ob = (int *)calloc((size_t)ng * 4 * (size_t)(c->nobs > 0 ? c->nobs : 1), sizeof(int));
+    int **ob = (int **)calloc((size_t)ng, sizeof(int *));   /* nob[ig] x {is, ie, js, je} */
+    int *nob = (int *)calloc((size_t)ng, sizeof(int));
     int *inl = (int *)calloc((size_t)ng * 2 * (size_t)(c->ninlt > 0 ? c->ninlt : 1), sizeof(int));
     int *outl = (int *)calloc((size_t)ng * 2 * (size_t)(c->noutlt > 0 ? c->noutlt : 1), sizeof(int));
     double **areax = (double **)calloc((size_t)ng, sizeof(double *));
@@ -102,7 +106,6 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
         l->dy = c->yl / (l->npy - 2);
         l->ax = l->dy;
         l->ay = l->dx;
-        l->has_obs = (c->nobs > 0 && ig < c->ig_obs);
         size_t n = (size_t)l->npx * (size_t)l->npy;
         l->u = (double *)malloc(n * sizeof(double));
         l->v = (double *)malloc(n * sizeof(double));
@@ -115,11 +118,18 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
         areax[ig] = (double *)malloc(n * sizeof(double));
         areay[ig] = (double *)malloc(n * sizeof(double));
     }
+    for (int ig = 0; ig < ng; ++ig) {
+        mgf_plan_level *l = &pl->lv[ig];
+        size_t cap = (c->nobs > 0 && ig >= c->ig_obs) ? (size_t)(l->npx - 2) * (size_t)(l->npy - 2)
+                                                       : (size_t)(c->nobs > 0 ? c->nobs : 1);
+        ob[ig] = (int *)calloc(4 * cap, sizeof(int));
+        if (c->nobs > 0 && ig < c->ig_obs) nob[ig] = c->nobs;
+    }
     for (int k = 0; k < c->nobs; ++k) {
         const int *o = c->obs + 4 * k;
         for (int ig = 0; ig < c->ig_obs; ++ig) {
             int fac = 1 << (c->ig_obs - 1 - ig);
-            int *r = ob + ((size_t)ig * (size_t)c->nobs + (size_t)k) * 4;
+            int *r = ob[ig] + (size_t)k * 4;
             r[0] = (o[0] + c->ibeg_obs - 1) * fac + 1;
             r[1] = (o[1] + c->ibeg_obs) * fac;
             r[2] = (o[2] - 1) * fac + 1;
@@ -131,6 +141,42 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
             }
         }
     }
+    /* Levels ig >= ig_obs, where the reference is undefined: a cell is an
+     * obstacle when all of its 2x2 children are, one rectangle per cell,
+     * i-major then j ascending (a subset of the finer obstacle region:
+     * blocking partly open cells would cut flow paths and can leave
+     * inlet-fed dead ends whose p' equation is singular). */
+    for (int ig = c->ig_obs; c->nobs > 0 && ig < ng; ++ig) {
+        mgf_plan_level *f = &pl->lv[ig - 1], *l = &pl->lv[ig];
+        int nxf = f->npx, nyf = f->npy;
+        unsigned char *blk = (unsigned char *)calloc((size_t)nxf * nyf, 1);
+        for (int k = 0; k < nob[ig - 1]; ++k) {
+            const int *r = ob[ig - 1] + (size_t)k * 4;
+            for (int j = r[2]; j <= r[3]; ++j)
+                for (int i = r[0]; i <= r[1]; ++i) blk[j * nxf + i] = 1;
+        }
+        int n = 0;
+        for (int ic = 1; ic <= l->npx - 2; ++ic)
+            for (int jc = 1; jc <= l->npy - 2; ++jc) {
+                int all = 1;
+                for (int dj = -1; dj <= 0; ++dj)
+                    for (int di = -1; di <= 0; ++di) {
+                        int fi = 2 * ic + di, fj = 2 * jc + dj;
+                        all &= (fi >= 1 && fi <= nxf - 2 && fj >= 1 && fj <= nyf - 2) &&
+                               blk[fj * nxf + fi];
+                    }
+                if (all) {
+                    int *r = ob[ig] + (size_t)n * 4;
+                    r[0] = ic; r[1] = ic; r[2] = jc; r[3] = jc;
+                    ++n;
+                }
+            }
+        nob[ig] = n;
+        l->ext = 1;
+        free(blk);
+    }
+    for (int ig = 0; ig < ng; ++ig) pl->lv[ig].has_obs = nob[ig] > 0;
+
     for (int k = 0; k < c->ninlt; ++k)
         for (int ig = 0; ig < c->ig_inlt; ++ig) {
             int fac = 1 << (c->ig_inlt - 1 - ig);
@@ -191,8 +237,8 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
         mgf_plan_level *l = &pl->lv[ig];
         if (!l->has_obs) continue;
         int nx = l->npx;
-        for (int k = 0; k < c->nobs; ++k) {
-            const int *r = ob + ((size_t)ig * (size_t)c->nobs + (size_t)k) * 4;
+        for (int k = 0; k < nob[ig]; ++k) {
+            const int *r = ob[ig] + (size_t)k * 4;
             for (int j = r[2]; j <= r[3]; ++j)
                 for (int i = r[0]; i <= r[1]; ++i) {
                     int ij = j * nx + i;
@@ -287,8 +333,8 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
         mgf_plan_level *l = &pl->lv[ig];
         if (!l->has_obs) continue;
         int nx = l->npx, ny = l->npy;
-        for (int k = 0; k < c->nobs; ++k) {
-            const int *r = ob + ((size_t)ig * (size_t)c->nobs + (size_t)k) * 4;
+        for (int k = 0; k < nob[ig]; ++k) {
+            const int *r = ob[ig] + (size_t)k * 4;
             for (int i = r[0]; i <= r[1]; ++i)
                 for (int j = r[2]; j <= r[3]; ++j) {
                     int ij = j * nx + i;
@@ -326,8 +372,8 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
         size_t n = (size_t)nx * l->npy;
         int *src = (int *)malloc(n * sizeof(int));
         for (size_t k = 0; k < n; ++k) src[k] = (int)k;
-        for (int k = 0; k < c->nobs; ++k) {
-            const int *r = ob + ((size_t)ig * (size_t)c->nobs + (size_t)k) * 4;
+        for (int k = 0; k < nob[ig]; ++k) {
+            const int *r = ob[ig] + (size_t)k * 4;
             int j = r[2];
             for (int i = r[0]; i <= r[1]; ++i) src[j * nx + i] = src[j * nx + i - nx];
             j = r[3];
@@ -357,7 +403,8 @@ done:
         free(areax[ig]);
         free(areay[ig]);
     }
-    free(areax); free(areay); free(ob); free(inl); free(outl);
+    for (int ig = 0; ig < ng; ++ig) free(ob[ig]);
+    free(areax); free(areay); free(ob); free(nob); free(inl); free(outl);
     if (rc) {
         mgf_plan_free(pl);
         return rc;

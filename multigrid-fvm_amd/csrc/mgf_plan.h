@@ -46,7 +46,8 @@ typedef struct mgf_plan_level {
     int npx, npy;
     double dx, dy;
     double ax, ay;             /* uniform face areas: areax = dy, areay = dx */
-    int has_obs;               /* obstacles act on this level (ig < ig_obs) */
+    int has_obs;               /* obstacles act on this level */
+    int ext;                   /* ig >= ig_obs: obstacles coarsened from the finer level */
     /* initial state after init + rest_geom, npx*npy each */
     double *u, *v, *p, *sc;
     uint8_t *flags;

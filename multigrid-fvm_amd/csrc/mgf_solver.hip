@@ -115,6 +115,7 @@ static int alloc_arena(mgf_solver *s)
         L.npy = l->npy;
         L.npcopy = l->npcopy;
         L.has_obs = l->has_obs;
+        L.ext = l->ext;
         L.ax = l->ax;
         L.ay = l->ay;
         /* the reference evaluates these per point from areax/areay arrays

@@ -23,10 +23,14 @@
  *   - mathematics: closed forms and invariants in tests/test_oracle_math.py.
  *
  * Deliberate definitions where the reference is undefined:
- *   - levels ig >= ig_obs carry no obstacles (the reference leaves their
- *     obstacle index arrays uninitialised and reads them; V6:471, 939-963,
- *     1153-1177).  Equal to the reference whenever ig_obs == ngrid.
- *     Parity for levels >= ig_obs is pinned only by invariants.
+ *   - levels ig >= ig_obs (the reference leaves their obstacle index arrays
+ *     uninitialised and reads them; V6:471, 939-963, 1153-1177): a cell is
+ *     an obstacle when all of its children are; the restriction onto such a
+ *     level zeroes the fine residual at penalised points (penalty_mask) and
+ *     the prolongation from it drops corrections at its penalised points.
+ *     Equal to the reference whenever ig_obs == ngrid.  For levels >= ig_obs
+ *     this is pinned only by invariants (a converged state is a V-cycle
+ *     fixed point), not by the reference.
  *   - an optional per-row inlet u profile (level-0 ghost face) replaces the
  *     discrete-inlet format for long profiles; applied after init and before
  *     the coarse-geometry restriction so coarse inlets follow V6:1440-1446.
@@ -59,7 +63,9 @@ static int fail(int code, const char *msg)
 typedef struct {
     int npx, npy;
     double dx, dy, amu;
-    int has_obs;                     /* obstacle ranges valid on this level */
+    int has_obs;                     /* obstacles act on this level */
+    int nob;                         /* obstacle rectangles on this level */
+    int ext;                         /* level >= ig_obs: derived (coarsened) obstacles */
     int *isobs, *ieobs, *jsobs, *jeobs;
     int *jsinlt, *jeinlt, *jsoutlt, *jeoutlt;
     int *tag;
@@ -214,6 +220,7 @@ static int build_levels(orc_solver *s)
         for (int k = 0; k_dnames[k]; ++k) *darr(l, k_dnames[k]) = dalloc(n);
         l->tag = (int *)calloc(n, sizeof(int));
         int no = c->nobs > 0 ? c->nobs : 1;
+        if (c->nobs > 0 && ig >= c->ig_obs) no = (l->npx - 2) * (l->npy - 2);
         l->isobs = (int *)calloc((size_t)no, sizeof(int));
         l->ieobs = (int *)calloc((size_t)no, sizeof(int));
         l->jsobs = (int *)calloc((size_t)no, sizeof(int));
@@ -239,8 +246,42 @@ static int build_levels(orc_solver *s)
                 return fail(-5, "obstacle outside the interior");
         }
     }
-    for (int ig = 0; ig < c->ngrid; ++ig)
-        s->L[ig].has_obs = (c->nobs > 0 && ig < c->ig_obs);
+    for (int ig = 0; ig < c->ngrid && ig < c->ig_obs; ++ig) s->L[ig].nob = c->nobs;
+    /* Levels below the obstacle level (ig >= ig_obs), where the reference is
+     * undefined: a cell is an obstacle when all of its 2x2 children on the
+     * next finer level are, listed one rectangle per cell, i-major then j
+     * ascending.  The coarse obstacle region is thus a subset of the finer
+     * one (blocking a coarse cell that is partly open would cut flow paths
+     * the finer levels have and can leave inlet-fed dead ends, whose p'
+     * equation is singular). */
+    for (int ig = c->ig_obs; c->nobs > 0 && ig < c->ngrid; ++ig) {
+        lvl_t *f = &s->L[ig - 1], *l = &s->L[ig];
+        int nxf = f->npx, nyf = f->npy;
+        unsigned char *blk = (unsigned char *)calloc((size_t)nxf * nyf, 1);
+        for (int k = 0; k < f->nob; ++k)
+            for (int j = f->jsobs[k]; j <= f->jeobs[k]; ++j)
+                for (int i = f->isobs[k]; i <= f->ieobs[k]; ++i) blk[j * nxf + i] = 1;
+        int n = 0;
+        for (int ic = 1; ic <= l->npx - 2; ++ic)
+            for (int jc = 1; jc <= l->npy - 2; ++jc) {
+                int all = 1;
+                for (int dj = -1; dj <= 0; ++dj)
+                    for (int di = -1; di <= 0; ++di) {
+                        int fi = 2 * ic + di, fj = 2 * jc + dj;
+                        all &= (fi >= 1 && fi <= nxf - 2 && fj >= 1 && fj <= nyf - 2) &&
+                               blk[fj * nxf + fi];
+                    }
+                if (all) {
+                    l->isobs[n] = ic; l->ieobs[n] = ic;
+                    l->jsobs[n] = jc; l->jeobs[n] = jc;
+                    ++n;
+                }
+            }
+        l->nob = n;
+        l->ext = 1;
+        free(blk);
+    }
+    for (int ig = 0; ig < c->ngrid; ++ig) s->L[ig].has_obs = s->L[ig].nob > 0;
     /* discrete inlets/outlets, V6:481-502 */
     for (int k = 0; k < c->ninlt; ++k)
         for (int ig = 0; ig < c->ig_inlt; ++ig) {
@@ -313,7 +354,7 @@ static void init_state(orc_solver *s)
         lvl_t *l = &s->L[ig];
         if (!l->has_obs) continue;
         int nx = l->npx;
-        for (int k = 0; k < c->nobs; ++k)
+        for (int k = 0; k < l->nob; ++k)
             for (int j = l->jsobs[k]; j <= l->jeobs[k]; ++j)
                 for (int i = l->isobs[k]; i <= l->ieobs[k]; ++i) {
                     int ij = j * nx + i;
@@ -523,7 +564,7 @@ static void coefsc(lvl_t *l, const orc_config *c)
             l->ap[ij] = ((l->aw[ij] + l->ae[ij]) + l->as[ij]) + l->an[ij];
         }
     if (!l->has_obs) return;
-    for (int k = 0; k < c->nobs; ++k)            /* i outer, j inner */
+    for (int k = 0; k < l->nob; ++k)             /* i outer, j inner */
         for (int i = l->isobs[k]; i <= l->ieobs[k]; ++i)
             for (int j = l->jsobs[k]; j <= l->jeobs[k]; ++j) {
                 int ij = j * nx + i;
@@ -603,7 +644,7 @@ static void update(lvl_t *l, const orc_config *c)
             l->p[ij] = l->p[ij] + c->relxp * (l->pp[ij] - ppref);
         }
     if (!l->has_obs) return;
-    for (int k = 0; k < c->nobs; ++k) {
+    for (int k = 0; k < l->nob; ++k) {
         int j = l->jsobs[k];
         for (int i = l->isobs[k]; i <= l->ieobs[k]; ++i) l->p[j * nx + i] = l->p[j * nx + i - nx];
         j = l->jeobs[k];
@@ -736,8 +777,19 @@ static void restv(lvl_t *f, lvl_t *c)
 
 static void zero(double *a, size_t n) { for (size_t k = 0; k < n; ++k) a[k] = 0.0; }
 
+/* Restriction onto a level below the obstacle level (ig >= ig_obs, where
+ * the reference is undefined): the fine residual is evaluated with
+ * un-penalised coefficients (V6:1500-1508), so at penalised faces/cells it
+ * never vanishes -- there it is balanced by the 1e30 penalty.  It is zeroed
+ * at penalised points before it enters the coarse tau term, so a converged
+ * fine state is a V-cycle fixed point.  Restrictions between levels above
+ * ig_obs (all of them when ig_obs == ngrid) keep the reference's
+ * arithmetic exactly. */
+static int penalty_mask(const lvl_t *f, const lvl_t *c) { (void)f; return c->ext; }
+
 static void restru(lvl_t *f, lvl_t *c, const orc_config *cf)
 {
+    const int mask = penalty_mask(f, c);
     int nxf = f->npx, nyf = f->npy, nxc = c->npx, nyc = c->npy;
     coefu(f);
     zero(f->rs, (size_t)nxf * nyf);
@@ -750,6 +802,7 @@ static void restru(lvl_t *f, lvl_t *c, const orc_config *cf)
                           f->areax[ij] * (f->p[ij] - f->p[ij + 1])) + f->resu[ij]) +
                         (cf->dpdxm * f->areax[ij]) * f->areay[ij];
             if (f->areax[ij] == 0.0) f->rs[ij] = 0.0;
+            if (mask && f->spu[ij] > 1.0e10) f->rs[ij] = 0.0;
         }
     coefu(c);
     zero(c->rs, (size_t)nxc * nyc);
@@ -773,6 +826,7 @@ static void restru(lvl_t *f, lvl_t *c, const orc_config *cf)
 
 static void restrv(lvl_t *f, lvl_t *c, const orc_config *cf)
 {
+    const int mask = penalty_mask(f, c);
     int nxf = f->npx, nyf = f->npy, nxc = c->npx, nyc = c->npy;
     coefv(f);
     zero(f->rs, (size_t)nxf * nyf);
@@ -785,6 +839,7 @@ static void restrv(lvl_t *f, lvl_t *c, const orc_config *cf)
                           f->areay[ij] * (f->p[ij] - f->p[ij + nxf])) + f->resv[ij]) +
                         (cf->dpdym * f->areax[ij]) * f->areay[ij];
             if (f->areay[ij] == 0.0) f->rs[ij] = 0.0;
+            if (mask && f->spv[ij] > 1.0e10) f->rs[ij] = 0.0;
         }
     coefv(c);
     zero(c->rs, (size_t)nxc * nyc);
@@ -808,6 +863,7 @@ static void restrv(lvl_t *f, lvl_t *c, const orc_config *cf)
 
 static void restrsc(lvl_t *f, lvl_t *c, const orc_config *cf)
 {
+    const int mask = penalty_mask(f, c);
     int nxf = f->npx, nyf = f->npy, nxc = c->npx, nyc = c->npy;
     coefsc(f, cf);
     zero(f->rs, (size_t)nxf * nyf);
@@ -817,6 +873,7 @@ static void restrsc(lvl_t *f, lvl_t *c, const orc_config *cf)
             f->rs[ij] = (((((f->aw[ij] * f->sc[ij - 1] + f->ae[ij] * f->sc[ij + 1]) +
                             f->as[ij] * f->sc[ij - nxf]) + f->an[ij] * f->sc[ij + nxf]) -
                           f->ap[ij] * f->sc[ij]) + f->ressc[ij]);
+            if (mask && f->spsc[ij] > 1.0e10) f->rs[ij] = 0.0;
         }
     coefsc(c, cf);
     zero(c->rs, (size_t)nxc * nyc);
@@ -850,6 +907,7 @@ static void prolu(lvl_t *f, lvl_t *c)
             double Ru = (f->u[ij] * f->areax[ij] + f->u[ij - nxf] * f->areax[ij - nxf]) /
                         (c->areax[ijc] + 1.0E-30);
             c->c[ijc] = c->u[ijc] - Ru;
+            if (c->ext && c->spu[ijc] > 1.0e10) c->c[ijc] = 0.0;
         }
     const double f1 = 3. / 4., f2 = 1. / 4.;
     for (int jc = 1; jc < nyc - 1; ++jc)
@@ -878,6 +936,7 @@ static void prolv(lvl_t *f, lvl_t *c)
             double Rv = (f->v[ij] * f->areay[ij] + f->v[ij - 1] * f->areay[ij - 1]) /
                         (c->areay[ijc] + 1.0E-30);
             c->c[ijc] = c->v[ijc] - Rv;
+            if (c->ext && c->spv[ijc] > 1.0e10) c->c[ijc] = 0.0;
         }
     const double f1 = 3. / 4., f2 = 1. / 4.;
     for (int jc = 1; jc < nyc - 2; ++jc)
@@ -905,6 +964,7 @@ static void prol_cell(lvl_t *f, lvl_t *c, double *ff, const double *fc, int mask
             double R = 0.25 * (((ff[ij] + ff[ij - 1]) + ff[ijs]) + ff[ijs - 1]);
             c->c[ijc] = fc[ijc] - R;
             if (mask && f->spu[ij] > 1.0e10) c->c[ijc] = 0.0;
+            if (c->ext && c->spsc[ijc] > 1.0e10) c->c[ijc] = 0.0;
         }
     const double f1 = 9. / 16., f2 = 3. / 16., f3 = 3. / 16., f4 = 1. / 16.;
     const double *cc = c->c;

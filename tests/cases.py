@@ -40,7 +40,18 @@ CASES = {
     "cfg0_rb": channel_128_inline().replace(isolve=2),
 }
 
-SMALL = ["small_jac", "small_rb", "ibc2", "ibc3", "ibc4", "odd_sweeps", "inlets", "ibeg",
+# Cases the reference cannot run (ig_obs < ngrid reads uninitialised obstacle
+# ranges, Version-6-26-04.c:471, 939-963): GPU vs CPU restatement only.
+EXTENSION_CASES = {
+    "coarse_free": Problem(nfx=64, nfy=64, ngrid=4, ig_obs=2, isolve=2, nvcycle=40, nswpp=4,
+                           amuf=0.01, obstacles=random_mask(32, 0.85, 11)),
+    "coarse_free_jac": Problem(nfx=64, nfy=32, ngrid=3, ig_obs=2, isolve=1, nvcycle=25,
+                               xl=2.0, yl=0.5,
+                               obstacles=[r for r in random_mask(32, 0.8, 12) if r[3] <= 16][:40]),
+}
+CASES.update(EXTENSION_CASES)
+
+SMALL = ["coarse_free", "coarse_free_jac", "small_jac", "small_rb", "ibc2", "ibc3", "ibc4", "odd_sweeps", "inlets", "ibeg",
          "rand64"]
 
 

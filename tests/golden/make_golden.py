@@ -41,7 +41,7 @@ from mgfvm.inputs import random_1024, random_mask, Problem  # noqa: E402
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 
 # whole-solve cases: every named case plus the large oracle-safe random mask
-E2E = dict(cases.CASES)
+E2E = {k: v for k, v in cases.CASES.items() if k not in cases.EXTENSION_CASES}
 E2E["rand1024_l5"] = random_1024(ngrid=5).replace(nvcycle=3)
 E2E["rand256_l3"] = Problem(nfx=256, nfy=256, ngrid=3, ig_obs=3, isolve=2, nvcycle=15,
                             nswpp=6, amuf=0.005, obstacles=random_mask(64, 0.8, 1))
