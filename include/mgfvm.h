@@ -120,6 +120,14 @@ int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy);
 /* the solver's HIP stream (a hipStream_t), for callers that order work on it */
 void *mgf_stream(mgf_solver *s);
 
+/* Restore the initial state built by mgf_create, on the device, without a
+ * host transfer (a fresh solve from inputs already resident in HBM).  The
+ * reference has no counterpart: it solves once per process.  Asynchronous
+ * on the solver's stream. */
+int mgf_reset(mgf_solver *s);
+/* Wait for all work on the solver's stream. */
+int mgf_synchronize(mgf_solver *s);
+
 /* ---- reference-granular steps (parity and debugging) ------------------ */
 
 /* One SIMPLE relaxation of level ig: fluxes, solve_u, solve_v,

@@ -32,6 +32,7 @@ struct mgf_solver {
     void *arena;
     size_t arena_bytes;
     DevLevel lv[MGF_MAX_LEVELS];
+    double *init_state[MGF_MAX_LEVELS];   /* u, v, p, sc as built at creation */
     DevPhys phys;
     int isolve, nswpm, nswpp;
     DevCtl *ctl;            /* device */
@@ -87,7 +88,7 @@ static int alloc_arena(mgf_solver *s)
     for (int ig = 0; ig < s->ng; ++ig) {
         const mgf_plan_level *l = &pl->lv[ig];
         size_t n = (size_t)l->npx * l->npy;
-        total += 23 * align256(n * sizeof(double));
+        total += 27 * align256(n * sizeof(double));
         total += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
         total += align256(n);                                  /* flags */
         total += align256(n * sizeof(uint64_t));               /* sccode */
@@ -133,6 +134,7 @@ static int alloc_arena(mgf_solver *s)
                              &L.apu, &L.apv, &L.cx, &L.cy, &L.aw, &L.ae, &L.as, &L.an,
                              &L.ap, &L.su, &L.tmp, &L.cu, &L.cv, &L.cp, &L.csc};
         for (int k = 0; k < 23; ++k) *arrs[k] = (double *)take(nb);
+        s->init_state[ig] = (double *)take(4 * nb);
         L.partial = (double *)take((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
@@ -152,6 +154,10 @@ static int alloc_arena(mgf_solver *s)
         if (l->npcopy > 0)
             HIPCHK(s, hipMemcpyAsync((void *)L.porig, l->porig, (size_t)l->npcopy * sizeof(int),
                                      hipMemcpyHostToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(s->init_state[ig], l->u, nb, hipMemcpyHostToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(s->init_state[ig] + n, l->v, nb, hipMemcpyHostToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(s->init_state[ig] + 2 * n, l->p, nb, hipMemcpyHostToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(s->init_state[ig] + 3 * n, l->sc, nb, hipMemcpyHostToDevice, s->st));
         /* apu = apv = 1e30 everywhere (V6:568-569); their ghost entries are
          * never overwritten and make the p' boundary coefficients ~0 */
         mgf_launch_fill(L.apu, 1.0e30, n, s->st);
@@ -338,6 +344,38 @@ static int reset_ctl(mgf_solver *s, int cyc, double tol)
 static int read_ctl(mgf_solver *s)
 {
     HIPCHK(s, hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(s, hipStreamSynchronize(s->st));
+    return MGF_OK;
+}
+
+/* Restore the state mgf_create built, on the device (no host transfer).
+ * Everything a solve reads before writing is restored: u, v, p, sc (initial
+ * and boundary values), pp and the tau terms (zero), and the momentum
+ * diagonals (1e30; their ghost entries are never rewritten).  The other
+ * arrays are scratch that every step rewrites before it reads them. */
+extern "C" int mgf_reset(mgf_solver *s)
+{
+    if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
+    for (int ig = 0; ig < s->ng; ++ig) {
+        DevLevel &L = s->lv[ig];
+        size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(double);
+        HIPCHK(s, hipMemcpyAsync(L.u, s->init_state[ig], nb, hipMemcpyDeviceToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(L.v, s->init_state[ig] + n, nb, hipMemcpyDeviceToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(L.p, s->init_state[ig] + 2 * n, nb, hipMemcpyDeviceToDevice, s->st));
+        HIPCHK(s, hipMemcpyAsync(L.sc, s->init_state[ig] + 3 * n, nb, hipMemcpyDeviceToDevice, s->st));
+        mgf_launch_fill(L.pp, 0.0, n, s->st);
+        mgf_launch_fill(L.resu, 0.0, n, s->st);
+        mgf_launch_fill(L.resv, 0.0, n, s->st);
+        mgf_launch_fill(L.ressc, 0.0, n, s->st);
+        mgf_launch_fill(L.apu, 1.0e30, n, s->st);
+        mgf_launch_fill(L.apv, 1.0e30, n, s->st);
+    }
+    return check_launch(s);
+}
+
+extern "C" int mgf_synchronize(mgf_solver *s)
+{
+    if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
 }

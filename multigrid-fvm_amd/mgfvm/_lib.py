@@ -21,7 +21,7 @@ EXPORTS = [
     "mgf_abi_version", "mgf_create", "mgf_destroy", "mgf_last_error", "mgf_num_levels",
     "mgf_level_dims", "mgf_stream", "mgf_relax", "mgf_restrict", "mgf_prolong",
     "mgf_vcycle", "mgf_solve", "mgf_get_field", "mgf_set_field", "mgf_get_fields",
-    "mgf_time_kernel",
+    "mgf_time_kernel", "mgf_reset", "mgf_synchronize",
 ]
 
 PREC_FP64 = 0
@@ -93,6 +93,8 @@ def lib():
         L.mgf_get_field.argtypes = [vp, ip, cp, P_d]
         L.mgf_set_field.argtypes = [vp, ip, cp, P_d]
         L.mgf_time_kernel.argtypes = [vp, ip, ip, ip, P_d, P_d]
+        L.mgf_reset.argtypes = [vp]
+        L.mgf_synchronize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -183,6 +185,13 @@ class Solver:
     def set(self, ig: int, name: str, arr: np.ndarray):
         a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         self._check(lib().mgf_set_field(self.h, ig, name.encode(), _dptr(a)))
+
+    def reset(self):
+        """Back to the initial state, on the device (asynchronous)."""
+        self._check(lib().mgf_reset(self.h))
+
+    def synchronize(self):
+        self._check(lib().mgf_synchronize(self.h))
 
     def relax(self, ig: int, leg: int) -> float:
         e = ctypes.c_double()

@@ -140,3 +140,23 @@ def test_solve_twice_restarts_counting(tmp_path):
         np.testing.assert_allclose(hg, ho, rtol=HIST_RTOL, atol=0.0)
     bad = _diff_report(gpu, ora, range(ora.ngrid))
     assert not bad, "\n".join(bad)
+
+
+def test_reset_reproduces_fresh_solve(tmp_path):
+    """mgf_reset restores exactly the state mgf_create built."""
+    import mgfvm
+    pin, ppat = cases.write_case("coarse_free", str(tmp_path))
+    _say("[gpu] reset coarse_free")
+    a = mgfvm.Solver(pin, ppat)
+    h1, n1, c1 = a.solve(15, 0.0)
+    f1 = {(ig, n): a.get(ig, n) for ig in range(a.ngrid) for n in STATE}
+    a.reset()
+    h2, n2, c2 = a.solve(15, 0.0)
+    assert (n1, c1) == (n2, c2) and np.array_equal(h1, h2)
+    for (ig, n), want in f1.items():
+        assert np.array_equal(a.get(ig, n).view(np.uint64), want.view(np.uint64)), (ig, n)
+    b = mgfvm.Solver(pin, ppat)
+    b.solve(15, 0.0)
+    for ig in range(a.ngrid):
+        for n in STATE:
+            assert np.array_equal(a.get(ig, n).view(np.uint64), b.get(ig, n).view(np.uint64)), (ig, n)
