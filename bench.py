@@ -15,8 +15,8 @@ cannot represent the obstacles and the reference's algorithm diverges
 (7 levels) -- see DESIGN.md.
 
 Printed: one JSON line with the throughput, the roofline of the dominant
-kernel (one colour pass of the red-black p' sweep on level 0, HIP events on
-the solver's stream) and a CPU baseline (the reference program built from
+kernel (one red-black sweep of the p' equation on level 0, both colours in
+one launch, HIP events on the solver's stream) and a CPU baseline (the reference program built from
 its own source, single core, a bounded number of V-cycles of the same
 solve, scaled to the cycles the solve needs).
 
@@ -146,12 +146,12 @@ def main():
         elapsed = time.perf_counter() - t0
         if not conv:
             print(f"bench.py: workload did not converge in {nv} V-cycles", file=sys.stderr)
-        # dominant kernel: red-black colour pass of the p' sweep on level 0
+        # dominant kernel: one red-black sweep (both colours) of the p' equation on level 0
         us, nbytes = s.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, 200)
         achieved = nbytes / (us * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_rb (one colour of the red-black p' sweep, level 0)",
+                "kernel": "k_rb_sweep (one red-black sweep of the p' equation, level 0)",
                 "mean_us": round(us, 3), "bytes_per_launch": nbytes}
         s.close()
         cpu = None if args.no_cpu else cpu_baseline(prob, ncyc, work)

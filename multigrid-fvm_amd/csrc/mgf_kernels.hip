@@ -209,12 +209,10 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
     return t;
 }
 
-__global__ void __launch_bounds__(BX * BY) k_asm_p(DevLevel L, DevPhys P, DevCtl *ctl, int lev,
-                                                   int record, double *hist)
+__global__ void __launch_bounds__(BX * BY) k_asm_p(DevLevel L, DevPhys P, const DevCtl *ctl)
 {
     if (ctl_done(ctl)) return;
     __shared__ double lds[BY];
-    __shared__ int am_last;
     const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
     const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
     const int nx = L.npx;
@@ -240,26 +238,23 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(DevLevel L, DevPhys P, DevCtl
         L.ap[ij] = ap;
     }
     double bs = block_sum(mag, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) L.partial[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+}
+
+/* sum of the per-block partials in a fixed order (one workgroup), so the
+ * level's sum |div| is the same bits on every run */
+__global__ void __launch_bounds__(BX * BY) k_reduce_err(DevLevel L, DevCtl *ctl, int lev, int nblk,
+                                                        int record, double *hist)
+{
+    if (ctl_done(ctl)) return;
+    __shared__ double lds[BY];
     const int tid = threadIdx.y * BX + threadIdx.x;
-    const unsigned nblk = gridDim.x * gridDim.y;
-    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
-    if (tid == 0) {
-        L.partial[b] = bs;
-        __threadfence();
-        unsigned prev = atomicAdd(&ctl->counter[lev], 1u);
-        am_last = (prev == nblk - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    const volatile double *part = L.partial;
     double t = 0.0;
-    for (unsigned k = (unsigned)tid; k < nblk; k += BX * BY) t += part[k];
+    for (int k = tid; k < nblk; k += BX * BY) t += L.partial[k];
     double tot = block_sum(t, lds);
     if (tid == 0) {
         ctl->err[lev] = tot;
         if (record) hist[ctl->cyc] = tot;
-        ctl->counter[lev] = 0u;
     }
 }
 
@@ -297,21 +292,140 @@ __global__ void __launch_bounds__(BX * BY) k_copy_range(double *dst, const doubl
     dst[j * npx + i] = src[j * npx + i];
 }
 
-/* one colour of the red-black sweep, in place (V6:1308-1349) */
-__global__ void __launch_bounds__(BX * BY) k_rb(DevLevel L, double *phi, int nx, int ny,
-                                                int colour, const DevCtl *ctl)
+/* One whole red-black sweep (both colours, V6:1308-1349) per launch.
+ * A 64 x RB_TY tile is staged in LDS with a 2-point halo read from `src`
+ * (points outside the swept range from `base`, the field itself); the red
+ * points of the tile and of a 1-point ring around it are updated first (the
+ * ring values are recomputed by each neighbouring tile from the same old
+ * values, so they are the same bits), then the black points of the tile, and
+ * the tile is written to `dst`.  Reading `src` and writing `dst` (ping-pong)
+ * keeps every tile's inputs the sweep's old values, exactly as the
+ * reference's colour-by-colour in-place passes see them.
+ * Each thread owns one column and RB_TY/BY rows of the tile and loads the
+ * six coefficients of its points into registers together with the halo
+ * staging, so the sweep pays one memory latency, not one per colour. */
+#define RB_TY 16
+#define RB_Q (RB_TY / BY)
+#define RB_RING (2 * (BX + 2) + 2 * RB_TY)
+
+struct Pt {
+    double aw, ae, as, an, ap, su;
+};
+
+__device__ __forceinline__ void rb_ring_pos(int k, int &r, int &c)
+{
+    /* ring of the tile in LDS coordinates (tile occupies rows/cols 2..) */
+    if (k < BX + 2) { r = 1; c = 1 + k; }
+    else if (k < 2 * (BX + 2)) { r = RB_TY + 2; c = 1 + (k - (BX + 2)); }
+    else if (k < 2 * (BX + 2) + RB_TY) { r = 2 + (k - 2 * (BX + 2)); c = 1; }
+    else { r = 2 + (k - 2 * (BX + 2) - RB_TY); c = BX + 2; }
+}
+
+__global__ void __launch_bounds__(BX * BY) k_rb_sweep(DevLevel L, const double *src, double *dst,
+                                                      const double *base, int nx, int ny,
+                                                      const DevCtl *ctl)
 {
     if (ctl_done(ctl)) return;
-    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
-    if (j > ny - 1) return;
-    const int i0 = (((1 + j) & 1) == colour) ? 1 : 2;
-    const int i = i0 + 2 * (int)(blockIdx.x * BX + threadIdx.x);
-    if (i > nx - 1) return;
-    const int npx = L.npx;
-    const int ij = j * npx + i;
-    const double sum_nb = ((L.aw[ij] * phi[ij - 1] + L.ae[ij] * phi[ij + 1]) +
-                           L.as[ij] * phi[ij - npx]) + L.an[ij] * phi[ij + npx];
-    phi[ij] = (sum_nb + L.su[ij]) / L.ap[ij];
+    __shared__ double t[RB_TY + 4][BX + 4];
+    const int npx = L.npx, npy = L.npy;
+    const int i0 = 1 + (int)blockIdx.x * BX, j0 = 1 + (int)blockIdx.y * RB_TY;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    constexpr int NST = ((RB_TY + 4) * (BX + 4) + BX * BY - 1) / (BX * BY);
+
+    /* 1. issue every global load: halo staging, own points, one ring point */
+    double sv[NST];
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int k = tid + q * BX * BY;
+        double val = 0.0;
+        if (k < (RB_TY + 4) * (BX + 4)) {
+            const int r = k / (BX + 4), c = k % (BX + 4);
+            const int gi = i0 - 2 + c, gj = j0 - 2 + r;
+            if (gi >= 0 && gi < npx && gj >= 0 && gj < npy) {
+                const int g = gj * npx + gi;
+                const bool in = (gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1);
+                val = in ? src[g] : base[g];
+            }
+        }
+        sv[q] = val;
+    }
+    const int gi = i0 + threadIdx.x;
+    Pt own[RB_Q];
+    bool live[RB_Q];
+#pragma unroll
+    for (int q = 0; q < RB_Q; ++q) {
+        const int gj = j0 + threadIdx.y + BY * q;
+        live[q] = (gi <= nx - 1 && gj <= ny - 1);
+        if (live[q]) {
+            const int g = gj * npx + gi;
+            own[q] = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
+        } else {
+            own[q] = Pt{0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+        }
+    }
+    int rr = 0, rc = 0;
+    bool ring = false;
+    Pt rp = Pt{0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+    if (tid < RB_RING) {
+        rb_ring_pos(tid, rr, rc);
+        const int ri = i0 - 2 + rc, rj = j0 - 2 + rr;
+        ring = (((ri + rj) & 1) == 0) && ri >= 1 && ri <= nx - 1 && rj >= 1 && rj <= ny - 1;
+        if (ring) {
+            const int g = rj * npx + ri;
+            rp = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
+        }
+    }
+    /* 2. staging to LDS */
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int k = tid + q * BX * BY;
+        if (k < (RB_TY + 4) * (BX + 4)) t[k / (BX + 4)][k % (BX + 4)] = sv[q];
+    }
+    __syncthreads();
+    /* 3. red points: own and ring */
+    double res[RB_Q];
+#pragma unroll
+    for (int q = 0; q < RB_Q; ++q) {
+        const int r = 2 + threadIdx.y + BY * q, c = 2 + threadIdx.x;
+        const int gj = j0 + threadIdx.y + BY * q;
+        res[q] = t[r][c];
+        if (live[q] && ((gi + gj) & 1) == 0) {
+            const Pt &o = own[q];
+            const double sum_nb = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
+                                  o.an * t[r + 1][c];
+            res[q] = (sum_nb + o.su) / o.ap;
+        }
+    }
+    double rv = 0.0;
+    if (ring) {
+        const double sum_nb = ((rp.aw * t[rr][rc - 1] + rp.ae * t[rr][rc + 1]) +
+                               rp.as * t[rr - 1][rc]) + rp.an * t[rr + 1][rc];
+        rv = (sum_nb + rp.su) / rp.ap;
+    }
+    /* red updates read only black neighbours, so red results can be stored
+     * without a barrier in between */
+#pragma unroll
+    for (int q = 0; q < RB_Q; ++q) {
+        const int gj = j0 + threadIdx.y + BY * q;
+        if (live[q] && ((gi + gj) & 1) == 0) t[2 + threadIdx.y + BY * q][2 + threadIdx.x] = res[q];
+    }
+    if (ring) t[rr][rc] = rv;
+    __syncthreads();
+    /* 4. black points, then write the tile */
+#pragma unroll
+    for (int q = 0; q < RB_Q; ++q) {
+        const int r = 2 + threadIdx.y + BY * q, c = 2 + threadIdx.x;
+        const int gj = j0 + threadIdx.y + BY * q;
+        if (!live[q]) continue;
+        double val = res[q];
+        if (((gi + gj) & 1) == 1) {
+            const Pt &o = own[q];
+            const double sum_nb = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
+                                  o.an * t[r + 1][c];
+            val = (sum_nb + o.su) / o.ap;
+        }
+        dst[gj * npx + gi] = val;
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -667,8 +781,10 @@ void mgf_launch_asm_v(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hi
 void mgf_launch_asm_p(const DevLevel &L, const DevPhys &P, DevCtl *ctl, int lev, int record,
                       double *hist, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_asm_p, grid2d(L.npx - 2, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl,
-                       lev, record, hist);
+    dim3 g = grid2d(L.npx - 2, L.npy - 2);
+    hipLaunchKernelGGL(k_asm_p, g, dim3(BX, BY), 0, st, L, P, (const DevCtl *)ctl);
+    hipLaunchKernelGGL(k_reduce_err, dim3(1), dim3(BX, BY), 0, st, L, ctl, lev,
+                       (int)(g.x * g.y), record, hist);
 }
 
 void mgf_launch_asm_sc(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
@@ -702,11 +818,19 @@ void mgf_launch_smooth(const DevLevel &L, int field, int isolve, int nsw, const 
             hipLaunchKernelGGL(k_copy_range, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, phi,
                                cur, L.npx, nx, ny, ctl);
     } else if (isolve == 2) {
-        dim3 g((unsigned)(((nx - 1 + 1) / 2 + BX - 1) / BX), (unsigned)((ny - 1 + BY - 1) / BY));
+        dim3 g((unsigned)((nx - 1 + BX - 1) / BX), (unsigned)((ny - 1 + RB_TY - 1) / RB_TY));
+        const double *cur = phi;
+        double *nxt = L.tmp;
         for (int it = 0; it < nsw; ++it) {
-            hipLaunchKernelGGL(k_rb, g, dim3(BX, BY), 0, st, L, phi, nx, ny, 0, ctl);
-            hipLaunchKernelGGL(k_rb, g, dim3(BX, BY), 0, st, L, phi, nx, ny, 1, ctl);
+            hipLaunchKernelGGL(k_rb_sweep, g, dim3(BX, BY), 0, st, L, cur, nxt,
+                               (const double *)phi, nx, ny, ctl);
+            double *tt = (double *)cur;
+            cur = nxt;
+            nxt = tt;
         }
+        if (cur != phi)
+            hipLaunchKernelGGL(k_copy_range, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, phi,
+                               cur, L.npx, nx, ny, ctl);
     }
 }
 

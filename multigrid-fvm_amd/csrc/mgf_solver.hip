@@ -575,8 +575,8 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     if ((rc = check_launch(s))) return rc;
     int nx = field == 2 ? L.npx - 1 : L.npx - 2, ny = L.npy - 1;
     double nunk = (double)(nx - 1) * (double)(ny - 1);
-    /* launches per sweep: red + black (2) or one Jacobi sweep (1) */
-    int per_sweep = isolve == 2 ? 2 : 1;
+    /* one launch per sweep (Jacobi sweep, or both red-black colours) */
+    int per_sweep = 1;
     if (mean_us) *mean_us = 1000.0 * (double)ms / ((double)reps * per_sweep);
     if (bytes_per_launch) *bytes_per_launch = 8.0 * 8.0 * nunk / per_sweep;
     return MGF_OK;
