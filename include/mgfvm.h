@@ -153,6 +153,28 @@ int mgf_vcycle(mgf_solver *s, int icyc, double *mass_err);
 int mgf_solve(mgf_solver *s, int nvcycle, double tol, double *history,
               int *ncycles, int *converged);
 
+/* ---- batches of independent problems ----------------------------------- */
+
+/* One solver for n independent problems (obstacle patterns, Reynolds
+ * numbers, grid sizes) that share isolve, nswpm and nswpp.  Every launch of
+ * the V-cycle covers the same-depth level (counted from the coarsest) of
+ * every problem that has it; each problem keeps the reference's schedule,
+ * its own tolerance (convergence) and cycle limit (nvcycle) and stops on its
+ * own.  The reference runs one problem per process
+ * (run_patterns_and_save_vtk.sh:43-77 loops over patterns); this replaces
+ * that loop.  mgf_create() is the n = 1 case; the step-level calls above
+ * need n = 1. */
+int mgf_batch_create(const mgf_problem *const *problems, int n, const mgf_options *opt,
+                     mgf_solver **out);
+int mgf_batch_size(const mgf_solver *s);
+int mgf_batch_num_levels(const mgf_solver *s, int b);
+int mgf_batch_level_dims(const mgf_solver *s, int b, int ig, int *npx, int *npy);
+/* Solve every problem; ncycles[n] and converged[n] (either may be NULL). */
+int mgf_batch_solve(mgf_solver *s, int *ncycles, int *converged);
+/* Copy up to cap history entries of problem b; returns the cycles run. */
+int mgf_batch_history(mgf_solver *s, int b, double *history, int cap);
+int mgf_batch_get_field(mgf_solver *s, int b, int ig, const char *name, double *out);
+
 /* ---- data ----------------------------------------------------------------- */
 
 /* Copy a level array to/from a caller buffer of npx*npy doubles.  Names:

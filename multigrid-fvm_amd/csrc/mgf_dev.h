@@ -1,11 +1,16 @@
 /*
- * mgf_dev.h -- device-side data of one solver (internal, HIP C++).
+ * mgf_dev.h -- device-side data of one solve group (internal, HIP C++).
  *
- * Layout in HBM: one allocation per solver holding, per level, the
+ * Layout in HBM: per instance one allocation holding, per level, the
  * reference's row-major npx*npy arrays (ij = j*npx + i) in FP64, 256-byte
  * aligned.  The face areas are uniform per level (checked at set-up), so they
  * and every metric derived from them are scalars here; the 1e30 penalty
  * arrays and the obstacle tag are one flag byte per point.
+ *
+ * Every kernel works on a group of levels with the same shape -- one level
+ * of one instance, or the same-depth level of many instances of a batch --
+ * and reads the descriptor of its level from a device array indexed by
+ * blockIdx.z.
  */
 #ifndef MGF_DEV_H
 #define MGF_DEV_H
@@ -15,17 +20,38 @@
 
 #define MGF_MAX_LEVELS 24
 
+/* device-resident control block of one instance's solve */
+struct DevCtl {
+    int cyc;                 /* V-cycles finished */
+    int done;                /* stopping test fired: later kernels return at once */
+    int nonfinite;           /* a recorded residual was NaN or infinite */
+    int maxcyc;              /* cycle limit of this solve */
+    double tol;
+    double *hist;            /* residual history of this instance */
+    double err[MGF_MAX_LEVELS];   /* latest sum |div| per level */
+};
+
+struct DevPhys {
+    double relxm, relxm1, relxp, sc_obs;
+    int ibc[4];
+};
+
 struct DevLevel {
     int npx, npy;
+    int ig;               /* level index within its instance (0 = finest) */
     int npcopy;
     int has_obs;
     int ext;              /* level below the obstacle level: coarsened obstacles */
+    int nblk;             /* workgroups of the residual kernel (partials) */
+    int pad;
     /* metrics, each computed on the host with the reference's expression */
     double ax, ay;        /* areax (= dy), areay (= dx) */
     double du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, wall adds */
     double ds_x, ds_y;    /* amusc*(areax/dx), amusc*(areay/dy): coefsc and its adds */
     double axax, ayay;    /* areax*areax, areay*areay: coefp */
     double bodyx, bodyy;  /* (dpdxm*areax)*areay, (dpdym*areax)*areay */
+    DevPhys P;            /* the instance's physics / boundary types */
+    DevCtl *ctl;          /* the instance's control block */
     /* state */
     double *u, *v, *p, *pp, *sc;
     double *resu, *resv, *ressc;     /* FAS tau terms (zero on level 0) */
@@ -33,7 +59,7 @@ struct DevLevel {
     /* scratch shared by the equations, as in the reference */
     double *cx, *cy;
     double *aw, *ae, *as, *an, *ap, *su;
-    double *tmp;                     /* Jacobi ping-pong */
+    double *tmp;                     /* smoother ping-pong */
     double *cu, *cv, *cp, *csc;      /* coarse-grid corrections (ghosts stay 0) */
     double *partial;                 /* per-block partial sums of |div| */
     /* obstacle data */
@@ -44,39 +70,36 @@ struct DevLevel {
     double *pstash;
 };
 
-struct DevPhys {
-    double relxm, relxm1, relxp, sc_obs;
-    int ibc[4];
+/* a set of same-shape levels launched together */
+struct DevGroup {
+    const DevLevel *d;       /* device array of descriptors */
+    int count;
+    int npx, npy;
+    int max_npcopy;
+    int any_pbound_x, any_pbound_y, any_outflow;
 };
 
-/* device-resident control block of a solve */
-struct DevCtl {
-    int cyc;                 /* V-cycles finished */
-    int done;                /* stopping test fired: later kernels return at once */
-    int nonfinite;           /* a recorded residual was NaN or infinite */
-    int pad;
-    double tol;
-    unsigned int counter[MGF_MAX_LEVELS];   /* last-block reduction tickets */
-    double err[MGF_MAX_LEVELS];             /* latest sum |div| per level */
+/* a restriction / prolongation between two same-length groups */
+struct DevPair {
+    const DevLevel *f, *c;   /* device arrays, element k of f is the finer level of c[k] */
+    int count;
+    int npxf, npyf, npxc, npyc;
 };
 
 /* launch wrappers (mgf_kernels.hip) */
-void mgf_launch_fluxes(const DevLevel &L, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_asm_u(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_asm_v(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_asm_p(const DevLevel &L, const DevPhys &P, DevCtl *ctl, int lev, int record,
-                      double *hist, hipStream_t st);
-void mgf_launch_asm_sc(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st);
+void mgf_launch_fluxes(const DevGroup &G, hipStream_t st);
+void mgf_launch_asm_u(const DevGroup &G, hipStream_t st);
+void mgf_launch_asm_v(const DevGroup &G, hipStream_t st);
+void mgf_launch_asm_p(const DevGroup &G, int record, hipStream_t st);
+void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
 /* field: 0 u, 1 v, 2 pp, 3 sc */
-void mgf_launch_smooth(const DevLevel &L, int field, int isolve, int nsw, const DevCtl *ctl,
-                       hipStream_t st);
-void mgf_launch_update(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_pbound(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_outflow(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_restrict(const DevLevel &F, const DevLevel &C, const DevPhys &P,
-                         const DevCtl *ctl, hipStream_t st);
-void mgf_launch_prolong(const DevLevel &F, const DevLevel &C, const DevCtl *ctl, hipStream_t st);
-void mgf_launch_end_cycle(DevCtl *ctl, const double *hist, hipStream_t st);
+void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
+void mgf_launch_update(const DevGroup &G, hipStream_t st);
+void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
+void mgf_launch_outflow(const DevGroup &G, hipStream_t st);
+void mgf_launch_restrict(const DevPair &R, hipStream_t st);
+void mgf_launch_prolong(const DevPair &R, hipStream_t st);
+void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
 void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st);
 int mgf_partial_blocks(int npx, int npy);
 

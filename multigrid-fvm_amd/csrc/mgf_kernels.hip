@@ -47,7 +47,15 @@ struct FluxInl {
     __device__ double Y(int k) const { return (0.5 * v[k]) * ay; }
 };
 
-__device__ __forceinline__ bool ctl_done(const DevCtl *ctl) { return ctl && ctl->done; }
+/* descriptor of this workgroup's level, and the early exit once its
+ * instance's stopping test has fired */
+#define LEVEL(Ls)                                 \
+    const DevLevel &L = (Ls)[blockIdx.z];         \
+    if (L.ctl->done) return;
+#define PAIR(Fs, Cs)                              \
+    const DevLevel &F = (Fs)[blockIdx.z];         \
+    const DevLevel &C = (Cs)[blockIdx.z];         \
+    if (C.ctl->done) return;
 
 /* coefu + no-slip wall doubling at one u face (V6:825-858) */
 template <class FX>
@@ -134,9 +142,9 @@ static inline dim3 grid2d(int ni, int nj)
 /* ---------------------------------------------------------------------- */
 /* fluxes, V6:678-694: j in [0,npy-2], i in [0,npx-2]                       */
 /* ---------------------------------------------------------------------- */
-__global__ void __launch_bounds__(BX * BY) k_fluxes(DevLevel L, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_fluxes(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
     RANGE2D(0, L.npx - 2, 0, L.npy - 2);
     const int ij = j * L.npx + i;
     L.cx[ij] = (0.5 * L.u[ij]) * L.ax;
@@ -146,9 +154,10 @@ __global__ void __launch_bounds__(BX * BY) k_fluxes(DevLevel L, const DevCtl *ct
 /* ---------------------------------------------------------------------- */
 /* coefu + walls + umom, V6:741-767, 815-860: j in [1,npy-2], i in [1,npx-3] */
 /* ---------------------------------------------------------------------- */
-__global__ void __launch_bounds__(BX * BY) k_asm_u(DevLevel L, DevPhys P, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_asm_u(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    const DevPhys &P = L.P;
     RANGE2D(1, L.npx - 3, 1, L.npy - 2);
     const int ij = j * L.npx + i;
     Coef c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
@@ -163,9 +172,10 @@ __global__ void __launch_bounds__(BX * BY) k_asm_u(DevLevel L, DevPhys P, const 
 }
 
 /* coefv + side walls + vmom, V6:769-793, 966-1016: j in [1,npy-3], i in [1,npx-2] */
-__global__ void __launch_bounds__(BX * BY) k_asm_v(DevLevel L, DevPhys P, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_asm_v(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    const DevPhys &P = L.P;
     RANGE2D(1, L.npx - 2, 1, L.npy - 3);
     const int ij = j * L.npx + i;
     Coef c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
@@ -180,9 +190,10 @@ __global__ void __launch_bounds__(BX * BY) k_asm_v(DevLevel L, DevPhys P, const 
 }
 
 /* coefsc + obstacle adds + scalar, V6:795-812, 861-964: [1,npy-2]x[1,npx-2] */
-__global__ void __launch_bounds__(BX * BY) k_asm_sc(DevLevel L, DevPhys P, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_asm_sc(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    const DevPhys &P = L.P;
     RANGE2D(1, L.npx - 2, 1, L.npy - 2);
     const int ij = j * L.npx + i;
     const uint8_t fl = L.flags[ij];
@@ -209,9 +220,10 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
     return t;
 }
 
-__global__ void __launch_bounds__(BX * BY) k_asm_p(DevLevel L, DevPhys P, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    const DevPhys &P = L.P;
     __shared__ double lds[BY];
     const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
     const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
@@ -243,18 +255,19 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(DevLevel L, DevPhys P, const 
 
 /* sum of the per-block partials in a fixed order (one workgroup), so the
  * level's sum |div| is the same bits on every run */
-__global__ void __launch_bounds__(BX * BY) k_reduce_err(DevLevel L, DevCtl *ctl, int lev, int nblk,
-                                                        int record, double *hist)
+__global__ void __launch_bounds__(BX * BY) k_reduce_err(const DevLevel *Ls, int record)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
     __shared__ double lds[BY];
     const int tid = threadIdx.y * BX + threadIdx.x;
     double t = 0.0;
-    for (int k = tid; k < nblk; k += BX * BY) t += L.partial[k];
+    for (int k = tid; k < L.nblk; k += BX * BY) t += L.partial[k];
     double tot = block_sum(t, lds);
     if (tid == 0) {
-        ctl->err[lev] = tot;
-        if (record) hist[ctl->cyc] = tot;
+        L.ctl->err[L.ig] = tot;
+        /* the level-0 value of the down-leg is the instance's history entry
+         * (V6:721-724) */
+        if (record && L.ig == 0) L.ctl->hist[L.ctl->cyc] = tot;
     }
 }
 
@@ -263,14 +276,28 @@ __global__ void __launch_bounds__(BX * BY) k_reduce_err(DevLevel L, DevCtl *ctl,
 /* j in [1,ny-1], i in [1,nx-1] (V6:1265-1266, 1314-1315)                   */
 /* ---------------------------------------------------------------------- */
 
+/* the field a smoother works on and the reference's solver extents
+ * (V6:701-736: u (npx-2, npy-1), v (npx-1, npy-2), pp and sc (npx-1, npy-1)) */
+__device__ __forceinline__ double *field_of(const DevLevel &L, int field, int &nx, int &ny)
+{
+    switch (field) {
+    case 0: nx = L.npx - 2; ny = L.npy - 1; return L.u;
+    case 1: nx = L.npx - 1; ny = L.npy - 2; return L.v;
+    case 2: nx = L.npx - 1; ny = L.npy - 1; return L.pp;
+    default: nx = L.npx - 1; ny = L.npy - 1; return L.sc;
+    }
+}
+
 /* one Jacobi sweep src -> dst; neighbours outside the swept range are read
  * from `base` (the field itself), which is what the reference's full copy
  * of phi into phi_new provides (V6:1254).  wj = 1 (V6:1241). */
-__global__ void __launch_bounds__(BX * BY) k_jacobi(DevLevel L, const double *src, double *dst,
-                                                    const double *base, int nx, int ny,
-                                                    const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_jacobi(const DevLevel *Ls, int field, int parity)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    int nx, ny;
+    double *base = field_of(L, field, nx, ny);
+    const double *src = parity ? L.tmp : base;
+    double *dst = parity ? base : L.tmp;
     RANGE2D(1, nx - 1, 1, ny - 1);
     const int npx = L.npx;
     const int ij = j * npx + i;
@@ -284,12 +311,14 @@ __global__ void __launch_bounds__(BX * BY) k_jacobi(DevLevel L, const double *sr
     dst[ij] = (1.0 - wj) * src[ij] + wj * g;
 }
 
-__global__ void __launch_bounds__(BX * BY) k_copy_range(double *dst, const double *src, int npx,
-                                                        int nx, int ny, const DevCtl *ctl)
+/* after an odd number of ping-pong sweeps the result is in tmp */
+__global__ void __launch_bounds__(BX * BY) k_copy_range(const DevLevel *Ls, int field)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    int nx, ny;
+    double *dst = field_of(L, field, nx, ny);
     RANGE2D(1, nx - 1, 1, ny - 1);
-    dst[j * npx + i] = src[j * npx + i];
+    dst[j * L.npx + i] = L.tmp[j * L.npx + i];
 }
 
 /* One whole red-black sweep (both colours, V6:1308-1349) per launch.
@@ -321,11 +350,13 @@ __device__ __forceinline__ void rb_ring_pos(int k, int &r, int &c)
     else { r = 2 + (k - 2 * (BX + 2) - RB_TY); c = BX + 2; }
 }
 
-__global__ void __launch_bounds__(BX * BY) k_rb_sweep(DevLevel L, const double *src, double *dst,
-                                                      const double *base, int nx, int ny,
-                                                      const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int field, int parity)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    int nx, ny;
+    const double *base = field_of(L, field, nx, ny);
+    const double *src = parity ? L.tmp : base;
+    double *dst = parity ? (double *)base : L.tmp;
     __shared__ double t[RB_TY + 4][BX + 4];
     const int npx = L.npx, npy = L.npy;
     const int i0 = 1 + (int)blockIdx.x * BX, j0 = 1 + (int)blockIdx.y * RB_TY;
@@ -434,9 +465,10 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(DevLevel L, const double *
 
 /* freshly updated p of every copy origin, taken before the update overwrites
  * anything (the reference copies after its update loop, in list order) */
-__global__ void k_pstash(DevLevel L, DevPhys P, const DevCtl *ctl)
+__global__ void k_pstash(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    const DevPhys &P = L.P;
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (s >= L.npcopy) return;
     const int o = L.porig[s];
@@ -445,9 +477,10 @@ __global__ void k_pstash(DevLevel L, DevPhys P, const DevCtl *ctl)
     L.pstash[s] = in ? L.p[o] + P.relxp * (L.pp[o] - 0.0) : L.p[o];
 }
 
-__global__ void __launch_bounds__(BX * BY) k_update(DevLevel L, DevPhys P, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    const DevPhys &P = L.P;
     RANGE2D(1, L.npx - 2, 1, L.npy - 2);
     const int nx = L.npx;
     const int ij = j * nx + i;
@@ -463,9 +496,10 @@ __global__ void __launch_bounds__(BX * BY) k_update(DevLevel L, DevPhys P, const
 /* ---------------------------------------------------------------------- */
 /* periodic / outflow boundaries, V6:1179-1228                              */
 /* ---------------------------------------------------------------------- */
-__global__ void k_pbound_x(DevLevel L, const DevCtl *ctl)
+__global__ void k_pbound_x(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    if (L.P.ibc[0] != 3) return;
     const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j > L.npy - 2) return;
     const int ij = j * L.npx, ij1 = ij + L.npx - 3;
@@ -477,9 +511,10 @@ __global__ void k_pbound_x(DevLevel L, const DevCtl *ctl)
 
 /* the y-periodic branch indexes ij + ny - 3 (V6:1203), so its iterations
  * depend on each other; one thread runs them in the reference's order */
-__global__ void k_pbound_y(DevLevel L, const DevCtl *ctl)
+__global__ void k_pbound_y(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    if (L.P.ibc[2] != 3) return;
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int nx = L.npx, ny = L.npy;
     for (int i = 1; i < nx - 1; ++i) {
@@ -491,9 +526,10 @@ __global__ void k_pbound_y(DevLevel L, const DevCtl *ctl)
     }
 }
 
-__global__ void k_outflow(DevLevel L, const DevCtl *ctl)
+__global__ void k_outflow(const DevLevel *Ls)
 {
-    if (ctl_done(ctl)) return;
+    LEVEL(Ls);
+    if (L.P.ibc[1] != 4) return;
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j > L.npy - 1) return;
     const int ij = j * L.npx + L.npx - 1;
@@ -508,9 +544,9 @@ __global__ void k_outflow(DevLevel L, const DevCtl *ctl)
 /* ---------------------------------------------------------------------- */
 
 /* restv, V6:1354-1389: coarse [1,nyc-2]x[1,nxc-2] */
-__global__ void __launch_bounds__(BX * BY) k_restv(DevLevel F, DevLevel C, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const DevLevel *Cs)
 {
-    if (ctl_done(ctl)) return;
+    PAIR(Fs, Cs);
     const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
     const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
     if (ic > C.npx - 2 || jc > C.npy - 2) return;
@@ -571,9 +607,9 @@ __device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fi
 /* FAS tau terms resu/resv/ressc on the coarse level (V6:1547-1563,
  * 1632-1647, 1723-1739); the fine and coarse residuals are evaluated where
  * they are needed instead of through the rs scratch array */
-__global__ void __launch_bounds__(BX * BY) k_tau(DevLevel F, DevLevel C, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_tau(const DevLevel *Fs, const DevLevel *Cs)
 {
-    if (ctl_done(ctl)) return;
+    PAIR(Fs, Cs);
     const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
     const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
     const int nxc = C.npx, nyc = C.npy;
@@ -614,9 +650,9 @@ __global__ void __launch_bounds__(BX * BY) k_tau(DevLevel F, DevLevel C, const D
  * prolv 1838-1855, prolp 1911-1928, prolsc 1979-1995).  All four read only
  * pre-prolongation fine data, so one pass serves them all; the entries the
  * reference leaves at zero are never written here (zero since creation). */
-__global__ void __launch_bounds__(BX * BY) k_prol_c(DevLevel F, DevLevel C, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const DevLevel *Cs)
 {
-    if (ctl_done(ctl)) return;
+    PAIR(Fs, Cs);
     const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
     const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
     const int nxc = C.npx, nyc = C.npy, nxf = F.npx;
@@ -687,9 +723,9 @@ __device__ __forceinline__ double corr_cell(const double *c, int nxc, int ic, in
 /* every fine point gathers its corrections; odd faces, which the reference
  * updates twice (atomics in its GPU build), take the two halves in its CPU
  * loop order (smaller coarse index first) */
-__global__ void __launch_bounds__(BX * BY) k_prol_apply(DevLevel F, DevLevel C, const DevCtl *ctl)
+__global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, const DevLevel *Cs)
 {
-    if (ctl_done(ctl)) return;
+    PAIR(Fs, Cs);
     const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
     const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
     const int nxf = F.npx, nyf = F.npy, nxc = C.npx, nyc = C.npy;
@@ -735,14 +771,17 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(DevLevel F, DevLevel C, 
 /* ---------------------------------------------------------------------- */
 /* end of a V-cycle: stopping test of V6:247-248                            */
 /* ---------------------------------------------------------------------- */
-__global__ void k_end_cycle(DevCtl *ctl, const double *hist)
+__global__ void k_end_cycle(DevCtl *ctls, int n)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= n) return;
+    DevCtl *ctl = &ctls[b];
     if (ctl->done) return;
-    const double h = hist[ctl->cyc];
+    const double h = ctl->hist[ctl->cyc];
     if (!(h == h) || h == __builtin_huge_val() || h == -__builtin_huge_val()) ctl->nonfinite = 1;
     if (h < ctl->tol) ctl->done = 1;
     ctl->cyc = ctl->cyc + 1;
+    if (ctl->cyc >= ctl->maxcyc) ctl->done = 1;
 }
 
 __global__ void k_fill(double *a, double v, size_t n)
@@ -752,7 +791,7 @@ __global__ void k_fill(double *a, double v, size_t n)
 }
 
 /* ====================================================================== */
-/* launch wrappers                                                         */
+/* launch wrappers: one launch per group, blockIdx.z = member of the group */
 /* ====================================================================== */
 
 int mgf_partial_blocks(int npx, int npy)
@@ -761,120 +800,105 @@ int mgf_partial_blocks(int npx, int npy)
     return (int)(g.x * g.y);
 }
 
-void mgf_launch_fluxes(const DevLevel &L, const DevCtl *ctl, hipStream_t st)
+static inline dim3 grid3d(int ni, int nj, int nz, int by = BY)
 {
-    hipLaunchKernelGGL(k_fluxes, grid2d(L.npx - 1, L.npy - 1), dim3(BX, BY), 0, st, L, ctl);
+    return dim3((unsigned)((ni + BX - 1) / BX), (unsigned)((nj + by - 1) / by), (unsigned)nz);
 }
 
-void mgf_launch_asm_u(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_fluxes(const DevGroup &G, hipStream_t st)
 {
-    if (L.npx - 3 < 1) return;
-    hipLaunchKernelGGL(k_asm_u, grid2d(L.npx - 3, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl);
+    hipLaunchKernelGGL(k_fluxes, grid3d(G.npx - 1, G.npy - 1, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
-void mgf_launch_asm_v(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_asm_u(const DevGroup &G, hipStream_t st)
 {
-    if (L.npy - 3 < 1) return;
-    hipLaunchKernelGGL(k_asm_v, grid2d(L.npx - 2, L.npy - 3), dim3(BX, BY), 0, st, L, P, ctl);
+    if (G.npx - 3 < 1) return;
+    hipLaunchKernelGGL(k_asm_u, grid3d(G.npx - 3, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
-void mgf_launch_asm_p(const DevLevel &L, const DevPhys &P, DevCtl *ctl, int lev, int record,
-                      double *hist, hipStream_t st)
+void mgf_launch_asm_v(const DevGroup &G, hipStream_t st)
 {
-    dim3 g = grid2d(L.npx - 2, L.npy - 2);
-    hipLaunchKernelGGL(k_asm_p, g, dim3(BX, BY), 0, st, L, P, (const DevCtl *)ctl);
-    hipLaunchKernelGGL(k_reduce_err, dim3(1), dim3(BX, BY), 0, st, L, ctl, lev,
-                       (int)(g.x * g.y), record, hist);
+    if (G.npy - 3 < 1) return;
+    hipLaunchKernelGGL(k_asm_v, grid3d(G.npx - 2, G.npy - 3, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
-void mgf_launch_asm_sc(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_asm_p(const DevGroup &G, int record, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_asm_sc, grid2d(L.npx - 2, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl);
+    hipLaunchKernelGGL(k_asm_p, grid3d(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_reduce_err, dim3(1, 1, (unsigned)G.count), dim3(BX, BY), 0, st, G.d,
+                       record);
 }
 
-void mgf_launch_smooth(const DevLevel &L, int field, int isolve, int nsw, const DevCtl *ctl,
-                       hipStream_t st)
+void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st)
 {
-    double *phi;
+    hipLaunchKernelGGL(k_asm_sc, grid3d(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+}
+
+void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
+{
     int nx, ny;
     switch (field) {
-    case 0: phi = L.u; nx = L.npx - 2; ny = L.npy - 1; break;
-    case 1: phi = L.v; nx = L.npx - 1; ny = L.npy - 2; break;
-    case 2: phi = L.pp; nx = L.npx - 1; ny = L.npy - 1; break;
-    default: phi = L.sc; nx = L.npx - 1; ny = L.npy - 1; break;
+    case 0: nx = G.npx - 2; ny = G.npy - 1; break;
+    case 1: nx = G.npx - 1; ny = G.npy - 2; break;
+    default: nx = G.npx - 1; ny = G.npy - 1; break;
     }
-    if (nx - 1 < 1 || ny - 1 < 1 || nsw <= 0) return;
-    if (isolve == 1) {
-        const double *cur = phi;
-        double *nxt = L.tmp;
-        for (int it = 0; it < nsw; ++it) {
-            hipLaunchKernelGGL(k_jacobi, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, L, cur, nxt,
-                               (const double *)phi, nx, ny, ctl);
-            double *t = (double *)cur;
-            cur = nxt;
-            nxt = t;
-        }
-        if (cur != phi)
-            hipLaunchKernelGGL(k_copy_range, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, phi,
-                               cur, L.npx, nx, ny, ctl);
-    } else if (isolve == 2) {
-        dim3 g((unsigned)((nx - 1 + BX - 1) / BX), (unsigned)((ny - 1 + RB_TY - 1) / RB_TY));
-        const double *cur = phi;
-        double *nxt = L.tmp;
-        for (int it = 0; it < nsw; ++it) {
-            hipLaunchKernelGGL(k_rb_sweep, g, dim3(BX, BY), 0, st, L, cur, nxt,
-                               (const double *)phi, nx, ny, ctl);
-            double *tt = (double *)cur;
-            cur = nxt;
-            nxt = tt;
-        }
-        if (cur != phi)
-            hipLaunchKernelGGL(k_copy_range, grid2d(nx - 1, ny - 1), dim3(BX, BY), 0, st, phi,
-                               cur, L.npx, nx, ny, ctl);
+    if (nx - 1 < 1 || ny - 1 < 1 || nsw <= 0 || (isolve != 1 && isolve != 2)) return;
+    for (int it = 0; it < nsw; ++it) {
+        if (isolve == 1)
+            hipLaunchKernelGGL(k_jacobi, grid3d(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
+                               field, it & 1);
+        else
+            hipLaunchKernelGGL(k_rb_sweep, grid3d(nx - 1, ny - 1, G.count, RB_TY), dim3(BX, BY),
+                               0, st, G.d, field, it & 1);
     }
+    if (nsw & 1)
+        hipLaunchKernelGGL(k_copy_range, grid3d(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                           G.d, field);
 }
 
-void mgf_launch_update(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_update(const DevGroup &G, hipStream_t st)
 {
-    if (L.npcopy > 0)
-        hipLaunchKernelGGL(k_pstash, dim3((unsigned)((L.npcopy + 255) / 256)), dim3(256), 0, st, L,
-                           P, ctl);
-    hipLaunchKernelGGL(k_update, grid2d(L.npx - 2, L.npy - 2), dim3(BX, BY), 0, st, L, P, ctl);
+    if (G.max_npcopy > 0)
+        hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
+                                          (unsigned)G.count),
+                           dim3(256), 0, st, G.d);
+    hipLaunchKernelGGL(k_update, grid3d(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
-void mgf_launch_pbound(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_pbound(const DevGroup &G, hipStream_t st)
 {
-    if (P.ibc[0] == 3)
-        hipLaunchKernelGGL(k_pbound_x, dim3((unsigned)((L.npy + 255) / 256)), dim3(256), 0, st, L,
-                           ctl);
-    if (P.ibc[2] == 3) hipLaunchKernelGGL(k_pbound_y, dim3(1), dim3(64), 0, st, L, ctl);
+    if (G.any_pbound_x)
+        hipLaunchKernelGGL(k_pbound_x, dim3((unsigned)((G.npy + 255) / 256), 1, (unsigned)G.count),
+                           dim3(256), 0, st, G.d);
+    if (G.any_pbound_y)
+        hipLaunchKernelGGL(k_pbound_y, dim3(1, 1, (unsigned)G.count), dim3(64), 0, st, G.d);
 }
 
-void mgf_launch_outflow(const DevLevel &L, const DevPhys &P, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_outflow(const DevGroup &G, hipStream_t st)
 {
-    if (P.ibc[1] == 4)
-        hipLaunchKernelGGL(k_outflow, dim3((unsigned)((L.npy + 255) / 256)), dim3(256), 0, st, L,
-                           ctl);
+    if (G.any_outflow)
+        hipLaunchKernelGGL(k_outflow, dim3((unsigned)((G.npy + 255) / 256), 1, (unsigned)G.count),
+                           dim3(256), 0, st, G.d);
 }
 
-void mgf_launch_restrict(const DevLevel &F, const DevLevel &C, const DevPhys &P,
-                         const DevCtl *ctl, hipStream_t st)
+void mgf_launch_restrict(const DevPair &R, hipStream_t st)
 {
-    (void)P;
-    dim3 g = grid2d(C.npx - 2, C.npy - 2);
-    hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, F, C, ctl);
-    hipLaunchKernelGGL(k_tau, g, dim3(BX, BY), 0, st, F, C, ctl);
+    dim3 g = grid3d(R.npxc - 2, R.npyc - 2, R.count);
+    hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, R.f, R.c);
+    hipLaunchKernelGGL(k_tau, g, dim3(BX, BY), 0, st, R.f, R.c);
 }
 
-void mgf_launch_prolong(const DevLevel &F, const DevLevel &C, const DevCtl *ctl, hipStream_t st)
+void mgf_launch_prolong(const DevPair &R, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_prol_c, grid2d(C.npx - 2, C.npy - 2), dim3(BX, BY), 0, st, F, C, ctl);
-    hipLaunchKernelGGL(k_prol_apply, grid2d(F.npx - 2, F.npy - 2), dim3(BX, BY), 0, st, F, C, ctl);
+    hipLaunchKernelGGL(k_prol_c, grid3d(R.npxc - 2, R.npyc - 2, R.count), dim3(BX, BY), 0, st,
+                       R.f, R.c);
+    hipLaunchKernelGGL(k_prol_apply, grid3d(R.npxf - 2, R.npyf - 2, R.count), dim3(BX, BY), 0,
+                       st, R.f, R.c);
 }
 
-void mgf_launch_end_cycle(DevCtl *ctl, const double *hist, hipStream_t st)
+void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_end_cycle, dim3(1), dim3(64), 0, st, ctl, hist);
+    hipLaunchKernelGGL(k_end_cycle, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, ctls, n);
 }
 
 void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st)

@@ -1,15 +1,22 @@
 /*
  * mgf_solver.hip -- solver objects behind the C ABI of include/mgfvm.h.
  *
- * One solver = one problem on one device: a single HBM allocation (the
- * "arena") holding every level's arrays, one HIP stream, a device control
- * block, the residual history, and (once a solve starts) one captured
- * V-cycle graph that is replayed until the device-side stopping test fires.
+ * A solver holds one or more independent problems ("instances") on one
+ * device: one HBM allocation (the "arena") with every level of every
+ * instance, one HIP stream, one device control block and residual history
+ * per instance, and (once a solve starts) one captured V-cycle graph that is
+ * replayed until every instance's device-side stopping test has fired.
  *
- * The schedule is the reference's, Version-6-26-04.c:195-249 (relax every
- * level on the way down with FAS restriction in between, prolongate on the
- * way up with post-relaxation of every level except 0, stop when the
- * level-0 sum of |div| recorded in this cycle is below the tolerance).
+ * The V-cycle is the reference's schedule, Version-6-26-04.c:195-249, run
+ * for all instances at once: levels are indexed by their depth above the
+ * coarsest level, every launch covers the same-depth, same-shape level of
+ * every instance that has it (blockIdx.z picks the instance), and the order
+ * of steps within each instance is exactly the reference's:
+ *   down: relax(d) then restrict(d -> d-1), d from the finest to 0;
+ *   up:   prolong(d-1 -> d), then relax(d) unless d is the instance's finest
+ *         level (level 0 gets no post-relaxation, V6:235);
+ *   end:  stop an instance when its level-0 sum |div| recorded in this cycle
+ *         is below its tolerance (V6:247-248) or its cycle limit is reached.
  */
 #include <hip/hip_runtime.h>
 
@@ -19,29 +26,50 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <vector>
+
 #include "mgf_dev.h"
 #include "mgf_plan.h"
 #include "mgfvm.h"
 
-struct mgf_solver {
+struct Inst {
     mgf_plan *plan;
-    mgf_options opt;
-    int device;
     int ng;
-    hipStream_t st;
-    void *arena;
-    size_t arena_bytes;
-    DevLevel lv[MGF_MAX_LEVELS];
+    DevLevel lv[MGF_MAX_LEVELS];          /* host copies of the descriptors */
     double *init_state[MGF_MAX_LEVELS];   /* u, v, p, sc as built at creation */
-    DevPhys phys;
-    int isolve, nswpm, nswpp;
-    DevCtl *ctl;            /* device */
-    DevCtl *ctl_host;       /* pinned mirror */
-    double *hist;           /* device, hist_cap entries */
-    int hist_cap;
-    hipGraphExec_t gexec;
-    int graph_ready;
-    char err[512];
+    int nvcycle;
+    double tol;
+};
+
+enum OpKind { OP_RELAX_DOWN, OP_RELAX_UP, OP_RESTRICT, OP_PROLONG };
+struct Op {
+    OpKind kind;
+    int g;                                /* group or pair index */
+};
+
+struct mgf_solver {
+    int nb = 0;
+    std::vector<Inst> inst;
+    mgf_options opt{};
+    int device = 0;
+    hipStream_t st = nullptr;
+    int isolve = 0, nswpm = 0, nswpp = 0;
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    DevLevel *d_desc = nullptr;           /* all group descriptor arrays */
+    std::vector<DevGroup> groups;
+    std::vector<DevPair> pairs;
+    std::vector<Op> ops;                  /* one V-cycle */
+    /* single-instance step API: group of level ig, pair with fine level ig */
+    std::vector<int> level_group;
+    std::vector<int> level_pair;
+    DevCtl *d_ctl = nullptr;              /* nb control blocks */
+    DevCtl *h_ctl = nullptr;              /* pinned mirror */
+    double *d_hist = nullptr;             /* nb x hist_cap */
+    int hist_cap = 0;
+    hipGraphExec_t gexec = nullptr;
+    int graph_ready = 0;
+    char err[512] = {0};
 };
 
 static int set_err(mgf_solver *s, int code, const char *fmt, ...)
@@ -75,48 +103,52 @@ static int check_launch(mgf_solver *s)
     return MGF_OK;
 }
 
-/* ---------------------------------------------------------------------- */
-/* creation                                                                 */
-/* ---------------------------------------------------------------------- */
-
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static int alloc_arena(mgf_solver *s)
-{
-    const mgf_plan *pl = s->plan;
-    size_t total = 0;
-    for (int ig = 0; ig < s->ng; ++ig) {
-        const mgf_plan_level *l = &pl->lv[ig];
-        size_t n = (size_t)l->npx * l->npy;
-        total += 27 * align256(n * sizeof(double));
-        total += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
-        total += align256(n);                                  /* flags */
-        total += align256(n * sizeof(uint64_t));               /* sccode */
-        total += align256(n * sizeof(int));                    /* pslot */
-        total += align256((size_t)(l->npcopy + 1) * sizeof(int));
-        total += align256((size_t)(l->npcopy + 1) * sizeof(double));
-    }
-    s->arena_bytes = total;
-    HIPCHK(s, hipMalloc(&s->arena, total));
-    HIPCHK(s, hipMemsetAsync(s->arena, 0, total, s->st));
+/* ---------------------------------------------------------------------- */
+/* arena                                                                    */
+/* ---------------------------------------------------------------------- */
 
-    char *cur = (char *)s->arena;
+static size_t level_bytes(const mgf_plan_level *l)
+{
+    size_t n = (size_t)l->npx * l->npy;
+    size_t t = 27 * align256(n * sizeof(double));  /* 23 arrays + 4 initial-state copies */
+    t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
+    t += align256(n);                               /* flags */
+    t += align256(n * sizeof(uint64_t));            /* sccode */
+    t += align256(n * sizeof(int));                 /* pslot */
+    t += align256((size_t)(l->npcopy + 1) * sizeof(int));
+    t += align256((size_t)(l->npcopy + 1) * sizeof(double));
+    return t;
+}
+
+static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl)
+{
+    const mgf_plan *pl = I.plan;
+    const mgf_problem *c = &pl->prob;
     auto take = [&](size_t bytes) {
         char *p = cur;
         cur += align256(bytes);
         return (void *)p;
     };
-    const mgf_problem *c = &pl->prob;
     const double amusc = c->amuf / c->prsc;
-    for (int ig = 0; ig < s->ng; ++ig) {
+    DevPhys P;
+    P.relxm = c->relxm;
+    P.relxm1 = 1.0 - c->relxm;
+    P.relxp = c->relxp;
+    P.sc_obs = c->sc_obs;
+    for (int k = 0; k < 4; ++k) P.ibc[k] = c->ibc[k];
+    for (int ig = 0; ig < I.ng; ++ig) {
         const mgf_plan_level *l = &pl->lv[ig];
-        DevLevel &L = s->lv[ig];
+        DevLevel &L = I.lv[ig];
         memset(&L, 0, sizeof L);
         L.npx = l->npx;
         L.npy = l->npy;
+        L.ig = ig;
         L.npcopy = l->npcopy;
         L.has_obs = l->has_obs;
         L.ext = l->ext;
+        L.nblk = mgf_partial_blocks(l->npx, l->npy);
         L.ax = l->ax;
         L.ay = l->ay;
         /* the reference evaluates these per point from areax/areay arrays
@@ -129,23 +161,28 @@ static int alloc_arena(mgf_solver *s)
         L.ayay = l->ay * l->ay;
         L.bodyx = (c->dpdxm * l->ax) * l->ay;
         L.bodyy = (c->dpdym * l->ax) * l->ay;
+        L.P = P;
+        L.ctl = ctl;
         size_t n = (size_t)l->npx * l->npy, nb = n * sizeof(double);
         double **arrs[23] = {&L.u, &L.v, &L.p, &L.pp, &L.sc, &L.resu, &L.resv, &L.ressc,
                              &L.apu, &L.apv, &L.cx, &L.cy, &L.aw, &L.ae, &L.as, &L.an,
                              &L.ap, &L.su, &L.tmp, &L.cu, &L.cv, &L.cp, &L.csc};
         for (int k = 0; k < 23; ++k) *arrs[k] = (double *)take(nb);
-        s->init_state[ig] = (double *)take(4 * nb);
-        L.partial = (double *)take((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
+        I.init_state[ig] = (double *)take(4 * nb);
+        L.partial = (double *)take((size_t)L.nblk * sizeof(double));
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
         L.pslot = (const int *)take(n * sizeof(int));
         L.porig = (const int *)take((size_t)(l->npcopy + 1) * sizeof(int));
         L.pstash = (double *)take((size_t)(l->npcopy + 1) * sizeof(double));
 
-        HIPCHK(s, hipMemcpyAsync(L.u, l->u, nb, hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(L.v, l->v, nb, hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(L.p, l->p, nb, hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(L.sc, l->sc, nb, hipMemcpyHostToDevice, s->st));
+        const double *src4[4] = {l->u, l->v, l->p, l->sc};
+        double *dst4[4] = {L.u, L.v, L.p, L.sc};
+        for (int k = 0; k < 4; ++k) {
+            HIPCHK(s, hipMemcpyAsync(dst4[k], src4[k], nb, hipMemcpyHostToDevice, s->st));
+            HIPCHK(s, hipMemcpyAsync(I.init_state[ig] + k * n, src4[k], nb,
+                                     hipMemcpyHostToDevice, s->st));
+        }
         HIPCHK(s, hipMemcpyAsync((void *)L.flags, l->flags, n, hipMemcpyHostToDevice, s->st));
         HIPCHK(s, hipMemcpyAsync((void *)L.sccode, l->sccode, n * sizeof(uint64_t),
                                  hipMemcpyHostToDevice, s->st));
@@ -154,10 +191,6 @@ static int alloc_arena(mgf_solver *s)
         if (l->npcopy > 0)
             HIPCHK(s, hipMemcpyAsync((void *)L.porig, l->porig, (size_t)l->npcopy * sizeof(int),
                                      hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(s->init_state[ig], l->u, nb, hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(s->init_state[ig] + n, l->v, nb, hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(s->init_state[ig] + 2 * n, l->p, nb, hipMemcpyHostToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(s->init_state[ig] + 3 * n, l->sc, nb, hipMemcpyHostToDevice, s->st));
         /* apu = apv = 1e30 everywhere (V6:568-569); their ghost entries are
          * never overwritten and make the p' boundary coefficients ~0 */
         mgf_launch_fill(L.apu, 1.0e30, n, s->st);
@@ -165,33 +198,164 @@ static int alloc_arena(mgf_solver *s)
         int rc = check_launch(s);
         if (rc) return rc;
     }
+    /* the pageable host plan arrays must stay alive until the copies finish */
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
 }
 
-static int ensure_hist(mgf_solver *s, int need)
+/* ---------------------------------------------------------------------- */
+/* groups and the V-cycle program                                           */
+/* ---------------------------------------------------------------------- */
+
+static void fill_group_meta(DevGroup &G, const std::vector<DevLevel> &m)
 {
-    if (need <= s->hist_cap) return MGF_OK;
-    int cap = need > 4096 ? need : 4096;
-    double *h = NULL;
-    HIPCHK(s, hipMalloc(&h, (size_t)cap * sizeof(double)));
-    HIPCHK(s, hipMemsetAsync(h, 0, (size_t)cap * sizeof(double), s->st));
-    if (s->hist) {
-        HIPCHK(s, hipStreamSynchronize(s->st));
-        (void)hipFree(s->hist);
+    G.count = (int)m.size();
+    G.npx = m[0].npx;
+    G.npy = m[0].npy;
+    G.max_npcopy = 0;
+    G.any_pbound_x = G.any_pbound_y = G.any_outflow = 0;
+    for (const DevLevel &L : m) {
+        if (L.npcopy > G.max_npcopy) G.max_npcopy = L.npcopy;
+        if (L.P.ibc[0] == 3) G.any_pbound_x = 1;
+        if (L.P.ibc[2] == 3) G.any_pbound_y = 1;
+        if (L.P.ibc[1] == 4) G.any_outflow = 1;
     }
-    s->hist = h;
-    s->hist_cap = cap;
-    if (s->graph_ready) {            /* the captured graph points at the old buffer */
-        (void)hipGraphExecDestroy(s->gexec);
-        s->graph_ready = 0;
+}
+
+/* Groups: at every depth d above the coarsest level, the levels of all
+ * instances that reach that depth, split by shape (kind 0: every such level,
+ * relaxed on the way down; kind 1: those that are not their instance's
+ * finest level, relaxed again on the way up).  Pairs: the same-shape levels
+ * at depth d >= 1 with their next coarser levels, for the transfers. */
+static int build_program(mgf_solver *s)
+{
+    int D = 0;
+    for (const Inst &I : s->inst) D = I.ng - 1 > D ? I.ng - 1 : D;
+    struct Key {
+        int d, nx, ny, kind;
+        bool operator==(const Key &o) const {
+            return d == o.d && nx == o.nx && ny == o.ny && kind == o.kind;
+        }
+    };
+    std::vector<std::vector<DevLevel>> gm;              /* group members */
+    std::vector<std::vector<DevLevel>> pf, pc;          /* pair members */
+    std::vector<std::pair<Key, int>> gkey, pkey;
+    auto find = [](const std::vector<std::pair<Key, int>> &v, const Key &k) {
+        for (const auto &e : v)
+            if (e.first == k) return e.second;
+        return -1;
+    };
+    std::vector<std::vector<int>> down_g(D + 1), up_g(D + 1), pr(D + 1);
+    for (int d = D; d >= 0; --d) {
+        for (int b = 0; b < s->nb; ++b) {
+            const Inst &I = s->inst[b];
+            const int ig = I.ng - 1 - d;
+            if (ig < 0) continue;
+            const DevLevel &L = I.lv[ig];
+            for (int kind = 0; kind < 2; ++kind) {
+                if (kind == 1 && ig == 0) continue;
+                Key k{d, L.npx, L.npy, kind};
+                int g = find(gkey, k);
+                if (g < 0) {
+                    g = (int)gm.size();
+                    gm.emplace_back();
+                    gkey.push_back({k, g});
+                    (kind == 0 ? down_g : up_g)[d].push_back(g);
+                }
+                gm[g].push_back(L);
+            }
+            if (d >= 1) {
+                Key k{d, L.npx, L.npy, 2};
+                int p = find(pkey, k);
+                if (p < 0) {
+                    p = (int)pf.size();
+                    pf.emplace_back();
+                    pc.emplace_back();
+                    pkey.push_back({k, p});
+                    pr[d].push_back(p);
+                }
+                pf[p].push_back(L);
+                pc[p].push_back(I.lv[ig + 1]);
+            }
+        }
+    }
+    std::vector<Op> ops;
+    for (int d = D; d >= 0; --d) {
+        for (int g : down_g[d]) ops.push_back({OP_RELAX_DOWN, g});
+        if (d >= 1)
+            for (int p : pr[d]) ops.push_back({OP_RESTRICT, p});
+    }
+    for (int d = 1; d <= D; ++d) {
+        for (int p : pr[d]) ops.push_back({OP_PROLONG, p});
+        for (int g : up_g[d]) ops.push_back({OP_RELAX_UP, g});
+    }
+    size_t ndesc = 0;
+    for (auto &g : gm) ndesc += g.size();
+    for (auto &p : pf) ndesc += 2 * p.size();
+    HIPCHK(s, hipMalloc(&s->d_desc, ndesc * sizeof(DevLevel)));
+    std::vector<DevLevel> host;
+    host.reserve(ndesc);
+    s->groups.assign(gm.size(), DevGroup());
+    s->pairs.assign(pf.size(), DevPair());
+    for (size_t g = 0; g < gm.size(); ++g) {
+        DevGroup &G = s->groups[g];
+        G.d = s->d_desc + host.size();
+        fill_group_meta(G, gm[g]);
+        host.insert(host.end(), gm[g].begin(), gm[g].end());
+    }
+    for (size_t p = 0; p < pf.size(); ++p) {
+        DevPair &R = s->pairs[p];
+        R.count = (int)pf[p].size();
+        R.npxf = pf[p][0].npx;
+        R.npyf = pf[p][0].npy;
+        R.npxc = pc[p][0].npx;
+        R.npyc = pc[p][0].npy;
+        for (const DevLevel &c : pc[p])
+            if (c.npx != R.npxc || c.npy != R.npyc)
+                return set_err(s, MGF_ERR_GEOMETRY, "coarse shapes differ within a transfer group");
+        R.f = s->d_desc + host.size();
+        host.insert(host.end(), pf[p].begin(), pf[p].end());
+        R.c = s->d_desc + host.size();
+        host.insert(host.end(), pc[p].begin(), pc[p].end());
+    }
+    HIPCHK(s, hipMemcpy(s->d_desc, host.data(), host.size() * sizeof(DevLevel),
+                        hipMemcpyHostToDevice));
+    s->ops = ops;
+    if (s->nb == 1) {
+        const Inst &I = s->inst[0];
+        s->level_group.assign(I.ng, -1);
+        s->level_pair.assign(I.ng, -1);
+        for (int ig = 0; ig < I.ng; ++ig) {
+            const int d = I.ng - 1 - ig;
+            s->level_group[ig] = down_g[d][0];
+            if (d >= 1) s->level_pair[ig] = pr[d][0];
+        }
     }
     return MGF_OK;
 }
 
-extern "C" int mgf_create(const mgf_problem *p, const mgf_options *opt, mgf_solver **out)
+/* ---------------------------------------------------------------------- */
+/* creation                                                                 */
+/* ---------------------------------------------------------------------- */
+
+static int ensure_hist(mgf_solver *s, int need)
 {
-    if (!p || !out) return set_err(NULL, MGF_ERR_ARG, "null argument");
+    if (need <= s->hist_cap) return MGF_OK;
+    const int cap = need > 4096 ? need : 4096;
+    double *h = NULL;
+    HIPCHK(s, hipStreamSynchronize(s->st));
+    HIPCHK(s, hipMalloc(&h, (size_t)cap * s->nb * sizeof(double)));
+    HIPCHK(s, hipMemset(h, 0, (size_t)cap * s->nb * sizeof(double)));
+    if (s->d_hist) (void)hipFree(s->d_hist);
+    s->d_hist = h;
+    s->hist_cap = cap;
+    return MGF_OK;
+}
+
+extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mgf_options *opt,
+                                mgf_solver **out)
+{
+    if (!probs || n < 1 || !out) return set_err(NULL, MGF_ERR_ARG, "bad arguments");
     *out = NULL;
     mgf_options o;
     if (opt)
@@ -200,68 +364,105 @@ extern "C" int mgf_create(const mgf_problem *p, const mgf_options *opt, mgf_solv
         mgf_options_default(&o);
     if (o.precision != MGF_PREC_FP64)
         return set_err(NULL, MGF_ERR_UNSUPPORTED, "precision mode %d not available", o.precision);
-    mgf_solver *s = (mgf_solver *)calloc(1, sizeof *s);
+    mgf_solver *s = new mgf_solver();
     s->opt = o;
+    s->nb = n;
+    s->inst.resize(n);
+    for (Inst &I : s->inst) {
+        I.plan = NULL;
+        I.ng = 0;
+    }
     char e[512] = {0};
-    int rc = mgf_plan_build(p, o.strict_limits, &s->plan, e, sizeof e);
-    if (rc) {
-        set_err(NULL, rc, "%s", e);
-        free(s);
-        return rc;
+    int rc = MGF_OK;
+    for (int b = 0; b < n; ++b) {
+        if (!probs[b]) {
+            mgf_destroy(s);
+            return set_err(NULL, MGF_ERR_ARG, "problem %d is null", b);
+        }
+        rc = mgf_plan_build(probs[b], o.strict_limits, &s->inst[b].plan, e, sizeof e);
+        if (rc) {
+            mgf_destroy(s);
+            if (n > 1) return set_err(NULL, rc, "problem %d: %s", b, e);
+            return set_err(NULL, rc, "%s", e);
+        }
+        Inst &I = s->inst[b];
+        I.ng = I.plan->ngrid;
+        I.nvcycle = I.plan->prob.nvcycle;
+        I.tol = I.plan->prob.convergence;
+        if (I.ng > MGF_MAX_LEVELS) {
+            mgf_destroy(s);
+            return set_err(NULL, MGF_ERR_LIMIT, "more than %d levels", MGF_MAX_LEVELS);
+        }
+        const mgf_problem *c = &I.plan->prob;
+        if (b == 0) {
+            s->isolve = c->isolve;
+            s->nswpm = c->nswpm;
+            s->nswpp = c->nswpp;
+        } else if (c->isolve != s->isolve || c->nswpm != s->nswpm || c->nswpp != s->nswpp) {
+            mgf_destroy(s);
+            return set_err(NULL, MGF_ERR_CONFIG,
+                           "batched problems must share isolve, nswpm and nswpp");
+        }
     }
-    s->ng = s->plan->ngrid;
-    if (s->ng > MGF_MAX_LEVELS) {
-        mgf_destroy(s);
-        return set_err(NULL, MGF_ERR_LIMIT, "more than %d levels", MGF_MAX_LEVELS);
-    }
-    const mgf_problem *c = &s->plan->prob;
-    s->isolve = c->isolve;
-    s->nswpm = c->nswpm;
-    s->nswpp = c->nswpp;
-    s->phys.relxm = c->relxm;
-    s->phys.relxm1 = 1.0 - c->relxm;
-    s->phys.relxp = c->relxp;
-    s->phys.sc_obs = c->sc_obs;
-    for (int k = 0; k < 4; ++k) s->phys.ibc[k] = c->ibc[k];
-
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         mgf_destroy(s);
         return set_err(NULL, MGF_ERR_DEVICE, "no HIP device available");
     }
-    if (o.device >= 0) {
-        if (hipSetDevice(o.device) != hipSuccess) {
-            mgf_destroy(s);
-            return set_err(NULL, MGF_ERR_DEVICE, "cannot select device %d", o.device);
-        }
+    if (o.device >= 0 && hipSetDevice(o.device) != hipSuccess) {
+        mgf_destroy(s);
+        return set_err(NULL, MGF_ERR_DEVICE, "cannot select device %d", o.device);
     }
     (void)hipGetDevice(&s->device);
     if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+        s->st = nullptr;
         mgf_destroy(s);
         return set_err(NULL, MGF_ERR_DEVICE, "stream creation failed");
     }
-    if ((rc = alloc_arena(s)) != MGF_OK) {
-        char keep[512];
-        snprintf(keep, sizeof keep, "%s", s->err);
+    size_t total = 0;
+    for (Inst &I : s->inst)
+        for (int ig = 0; ig < I.ng; ++ig) total += level_bytes(&I.plan->lv[ig]);
+    s->arena_bytes = total;
+    if (hipMalloc(&s->arena, total) != hipSuccess) {
+        s->arena = nullptr;
         mgf_destroy(s);
-        return set_err(NULL, rc, "%s", keep);
+        return set_err(NULL, MGF_ERR_DEVICE, "device allocation of %zu bytes failed", total);
     }
-    if (hipMalloc(&s->ctl, sizeof(DevCtl)) != hipSuccess ||
-        hipHostMalloc(&s->ctl_host, sizeof(DevCtl), 0) != hipSuccess) {
+    if (hipMalloc(&s->d_ctl, (size_t)n * sizeof(DevCtl)) != hipSuccess ||
+        hipHostMalloc(&s->h_ctl, (size_t)n * sizeof(DevCtl), 0) != hipSuccess) {
         mgf_destroy(s);
         return set_err(NULL, MGF_ERR_DEVICE, "control block allocation failed");
     }
-    memset(s->ctl_host, 0, sizeof(DevCtl));
-    (void)hipMemcpy(s->ctl, s->ctl_host, sizeof(DevCtl), hipMemcpyHostToDevice);
-    if ((rc = ensure_hist(s, c->nvcycle > 0 ? c->nvcycle : 1)) != MGF_OK) {
+    (void)hipMemsetAsync(s->arena, 0, total, s->st);
+    int maxnv = 1;
+    for (Inst &I : s->inst) maxnv = I.nvcycle > maxnv ? I.nvcycle : maxnv;
+    if ((rc = ensure_hist(s, maxnv)) == MGF_OK) {
+        char *cur = (char *)s->arena;
+        for (int b = 0; b < n && !rc; ++b) rc = upload_instance(s, s->inst[b], cur, s->d_ctl + b);
+    }
+    if (!rc) rc = build_program(s);
+    if (!rc) {
+        memset(s->h_ctl, 0, (size_t)n * sizeof(DevCtl));
+        for (int b = 0; b < n; ++b) s->h_ctl[b].hist = s->d_hist + (size_t)b * s->hist_cap;
+        if (hipMemcpy(s->d_ctl, s->h_ctl, (size_t)n * sizeof(DevCtl), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            rc = set_err(s, MGF_ERR_DEVICE, "control block upload failed");
+    }
+    if (rc) {
         char keep[512];
         snprintf(keep, sizeof keep, "%s", s->err);
         mgf_destroy(s);
         return set_err(NULL, rc, "%s", keep);
     }
-    (void)hipStreamSynchronize(s->st);
     *out = s;
     return MGF_OK;
+}
+
+extern "C" int mgf_create(const mgf_problem *p, const mgf_options *opt, mgf_solver **out)
+{
+    if (!p || !out) return set_err(NULL, MGF_ERR_ARG, "null argument");
+    const mgf_problem *one[1] = {p};
+    return mgf_batch_create(one, 1, opt, out);
 }
 
 extern "C" void mgf_destroy(mgf_solver *s)
@@ -270,80 +471,101 @@ extern "C" void mgf_destroy(mgf_solver *s)
     if (s->st) (void)hipStreamSynchronize(s->st);
     if (s->graph_ready) (void)hipGraphExecDestroy(s->gexec);
     if (s->arena) (void)hipFree(s->arena);
-    if (s->hist) (void)hipFree(s->hist);
-    if (s->ctl) (void)hipFree(s->ctl);
-    if (s->ctl_host) (void)hipHostFree(s->ctl_host);
+    if (s->d_desc) (void)hipFree(s->d_desc);
+    if (s->d_hist) (void)hipFree(s->d_hist);
+    if (s->d_ctl) (void)hipFree(s->d_ctl);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->st) (void)hipStreamDestroy(s->st);
-    mgf_plan_free(s->plan);
-    free(s);
+    for (Inst &I : s->inst) mgf_plan_free(I.plan);
+    delete s;
 }
 
 extern "C" const char *mgf_last_error(const mgf_solver *s) { return s ? s->err : mgf_error_string(); }
-extern "C" int mgf_num_levels(const mgf_solver *s) { return s ? s->ng : 0; }
+extern "C" int mgf_batch_size(const mgf_solver *s) { return s ? s->nb : 0; }
 extern "C" void *mgf_stream(mgf_solver *s) { return s ? (void *)s->st : NULL; }
+
+extern "C" int mgf_batch_num_levels(const mgf_solver *s, int b)
+{
+    return (s && b >= 0 && b < s->nb) ? s->inst[b].ng : 0;
+}
+
+extern "C" int mgf_num_levels(const mgf_solver *s) { return mgf_batch_num_levels(s, 0); }
+
+extern "C" int mgf_batch_level_dims(const mgf_solver *s, int b, int ig, int *npx, int *npy)
+{
+    if (!s || b < 0 || b >= s->nb || ig < 0 || ig >= s->inst[b].ng) return MGF_ERR_ARG;
+    if (npx) *npx = s->inst[b].lv[ig].npx;
+    if (npy) *npy = s->inst[b].lv[ig].npy;
+    return MGF_OK;
+}
 
 extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
 {
-    if (!s || ig < 0 || ig >= s->ng) return MGF_ERR_ARG;
-    if (npx) *npx = s->lv[ig].npx;
-    if (npy) *npy = s->lv[ig].npy;
-    return MGF_OK;
+    return mgf_batch_level_dims(s, 0, ig, npx, npy);
 }
 
 /* ---------------------------------------------------------------------- */
 /* schedule                                                                 */
 /* ---------------------------------------------------------------------- */
 
-/* V6:201-207 (leg 0) and V6:237-243 (leg 1) */
-static void enqueue_relax(mgf_solver *s, int ig, int leg, int record)
+/* V6:201-207 (leg 0) and V6:237-243 (leg 1) for a group */
+static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg)
 {
-    const DevLevel &L = s->lv[ig];
     hipStream_t st = s->st;
-    mgf_launch_fluxes(L, s->ctl, st);
-    mgf_launch_asm_u(L, s->phys, s->ctl, st);
-    mgf_launch_smooth(L, 0, s->isolve, s->nswpm, s->ctl, st);
-    mgf_launch_asm_v(L, s->phys, s->ctl, st);
-    mgf_launch_smooth(L, 1, s->isolve, s->nswpm, s->ctl, st);
-    mgf_launch_asm_p(L, s->phys, s->ctl, ig, record, s->hist, st);
-    mgf_launch_smooth(L, 2, s->isolve, s->nswpp, s->ctl, st);
-    mgf_launch_update(L, s->phys, s->ctl, st);
-    mgf_launch_pbound(L, s->phys, s->ctl, st);
+    mgf_launch_fluxes(G, st);
+    mgf_launch_asm_u(G, st);
+    mgf_launch_smooth(G, 0, s->isolve, s->nswpm, st);
+    mgf_launch_asm_v(G, st);
+    mgf_launch_smooth(G, 1, s->isolve, s->nswpm, st);
+    mgf_launch_asm_p(G, leg == 0, st);
+    mgf_launch_smooth(G, 2, s->isolve, s->nswpp, st);
+    mgf_launch_update(G, st);
+    mgf_launch_pbound(G, st);
     if (leg == 0) {
-        mgf_launch_outflow(L, s->phys, s->ctl, st);
-        mgf_launch_asm_sc(L, s->phys, s->ctl, st);
-        mgf_launch_smooth(L, 3, s->isolve, s->nswpm, s->ctl, st);
+        mgf_launch_outflow(G, st);
+        mgf_launch_asm_sc(G, st);
+        mgf_launch_smooth(G, 3, s->isolve, s->nswpm, st);
     } else {
-        mgf_launch_asm_sc(L, s->phys, s->ctl, st);
-        mgf_launch_smooth(L, 3, s->isolve, s->nswpm, s->ctl, st);
-        mgf_launch_outflow(L, s->phys, s->ctl, st);
+        mgf_launch_asm_sc(G, st);
+        mgf_launch_smooth(G, 3, s->isolve, s->nswpm, st);
+        mgf_launch_outflow(G, st);
     }
 }
 
 static void enqueue_vcycle(mgf_solver *s)
 {
-    const int ng = s->ng;
-    for (int ig = 0; ig < ng; ++ig) {
-        enqueue_relax(s, ig, 0, ig == 0);
-        if (ig != ng - 1) mgf_launch_restrict(s->lv[ig], s->lv[ig + 1], s->phys, s->ctl, s->st);
-    }
-    for (int ig = ng - 1; ig > 0; --ig) {
-        mgf_launch_prolong(s->lv[ig - 1], s->lv[ig], s->ctl, s->st);
-        if (ig != 1) enqueue_relax(s, ig - 1, 1, 0);
+    for (const Op &o : s->ops) {
+        switch (o.kind) {
+        case OP_RELAX_DOWN: enqueue_relax(s, s->groups[o.g], 0); break;
+        case OP_RELAX_UP: enqueue_relax(s, s->groups[o.g], 1); break;
+        case OP_RESTRICT: mgf_launch_restrict(s->pairs[o.g], s->st); break;
+        case OP_PROLONG: mgf_launch_prolong(s->pairs[o.g], s->st); break;
+        }
     }
 }
 
-static int reset_ctl(mgf_solver *s, int cyc, double tol)
+/* control blocks: every instance starts counting at cycle `cyc`, with its
+ * tolerance and cycle limit (a limit <= cyc marks it finished) */
+static int reset_ctl(mgf_solver *s, int cyc, const double *tol, const int *limit)
 {
-    memset(s->ctl_host, 0, sizeof(DevCtl));
-    s->ctl_host->cyc = cyc;
-    s->ctl_host->tol = tol;
-    HIPCHK(s, hipMemcpyAsync(s->ctl, s->ctl_host, sizeof(DevCtl), hipMemcpyHostToDevice, s->st));
+    for (int b = 0; b < s->nb; ++b) {
+        DevCtl &c = s->h_ctl[b];
+        memset(&c, 0, sizeof c);
+        c.cyc = cyc;
+        c.tol = tol ? tol[b] : 0.0;
+        c.maxcyc = limit ? limit[b] : 0x7fffffff;
+        c.done = c.maxcyc <= cyc;
+        c.hist = s->d_hist + (size_t)b * s->hist_cap;
+    }
+    HIPCHK(s, hipMemcpyAsync(s->d_ctl, s->h_ctl, (size_t)s->nb * sizeof(DevCtl),
+                             hipMemcpyHostToDevice, s->st));
     return MGF_OK;
 }
 
 static int read_ctl(mgf_solver *s)
 {
-    HIPCHK(s, hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(s, hipMemcpyAsync(s->h_ctl, s->d_ctl, (size_t)s->nb * sizeof(DevCtl),
+                             hipMemcpyDeviceToHost, s->st));
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
 }
@@ -356,20 +578,21 @@ static int read_ctl(mgf_solver *s)
 extern "C" int mgf_reset(mgf_solver *s)
 {
     if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
-    for (int ig = 0; ig < s->ng; ++ig) {
-        DevLevel &L = s->lv[ig];
-        size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(double);
-        HIPCHK(s, hipMemcpyAsync(L.u, s->init_state[ig], nb, hipMemcpyDeviceToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(L.v, s->init_state[ig] + n, nb, hipMemcpyDeviceToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(L.p, s->init_state[ig] + 2 * n, nb, hipMemcpyDeviceToDevice, s->st));
-        HIPCHK(s, hipMemcpyAsync(L.sc, s->init_state[ig] + 3 * n, nb, hipMemcpyDeviceToDevice, s->st));
-        mgf_launch_fill(L.pp, 0.0, n, s->st);
-        mgf_launch_fill(L.resu, 0.0, n, s->st);
-        mgf_launch_fill(L.resv, 0.0, n, s->st);
-        mgf_launch_fill(L.ressc, 0.0, n, s->st);
-        mgf_launch_fill(L.apu, 1.0e30, n, s->st);
-        mgf_launch_fill(L.apv, 1.0e30, n, s->st);
-    }
+    for (Inst &I : s->inst)
+        for (int ig = 0; ig < I.ng; ++ig) {
+            DevLevel &L = I.lv[ig];
+            size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(double);
+            double *dst4[4] = {L.u, L.v, L.p, L.sc};
+            for (int k = 0; k < 4; ++k)
+                HIPCHK(s, hipMemcpyAsync(dst4[k], I.init_state[ig] + k * n, nb,
+                                         hipMemcpyDeviceToDevice, s->st));
+            mgf_launch_fill(L.pp, 0.0, n, s->st);
+            mgf_launch_fill(L.resu, 0.0, n, s->st);
+            mgf_launch_fill(L.resv, 0.0, n, s->st);
+            mgf_launch_fill(L.ressc, 0.0, n, s->st);
+            mgf_launch_fill(L.apu, 1.0e30, n, s->st);
+            mgf_launch_fill(L.apv, 1.0e30, n, s->st);
+        }
     return check_launch(s);
 }
 
@@ -380,25 +603,37 @@ extern "C" int mgf_synchronize(mgf_solver *s)
     return MGF_OK;
 }
 
+static int single_only(mgf_solver *s)
+{
+    if (s->nb != 1)
+        return set_err(s, MGF_ERR_UNSUPPORTED, "step-level calls need a single-problem solver");
+    return MGF_OK;
+}
+
 extern "C" int mgf_relax(mgf_solver *s, int ig, int leg, double *mass_err)
 {
-    if (!s || ig < 0 || ig >= s->ng || (leg != 0 && leg != 1))
-        return set_err(s, MGF_ERR_ARG, "bad level or leg");
-    int rc = reset_ctl(s, 0, 0.0);
+    if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
+    int rc = single_only(s);
     if (rc) return rc;
-    enqueue_relax(s, ig, leg, 0);
+    if (ig < 0 || ig >= s->inst[0].ng || (leg != 0 && leg != 1))
+        return set_err(s, MGF_ERR_ARG, "bad level or leg");
+    if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
+    /* a leg-0 relaxation of level 0 also writes history slot 0 (unused here) */
+    enqueue_relax(s, s->groups[s->level_group[ig]], leg);
     if ((rc = check_launch(s))) return rc;
     if ((rc = read_ctl(s))) return rc;
-    if (mass_err) *mass_err = s->ctl_host->err[ig];
+    if (mass_err) *mass_err = s->h_ctl[0].err[ig];
     return MGF_OK;
 }
 
 extern "C" int mgf_restrict(mgf_solver *s, int ig)
 {
-    if (!s || ig < 0 || ig + 1 >= s->ng) return set_err(s, MGF_ERR_ARG, "bad fine level");
-    int rc = reset_ctl(s, 0, 0.0);
+    if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
+    int rc = single_only(s);
     if (rc) return rc;
-    mgf_launch_restrict(s->lv[ig], s->lv[ig + 1], s->phys, s->ctl, s->st);
+    if (ig < 0 || ig + 1 >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad fine level");
+    if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
+    mgf_launch_restrict(s->pairs[s->level_pair[ig]], s->st);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
@@ -406,10 +641,12 @@ extern "C" int mgf_restrict(mgf_solver *s, int ig)
 
 extern "C" int mgf_prolong(mgf_solver *s, int igc)
 {
-    if (!s || igc < 1 || igc >= s->ng) return set_err(s, MGF_ERR_ARG, "bad coarse level");
-    int rc = reset_ctl(s, 0, 0.0);
+    if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
+    int rc = single_only(s);
     if (rc) return rc;
-    mgf_launch_prolong(s->lv[igc - 1], s->lv[igc], s->ctl, s->st);
+    if (igc < 1 || igc >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad coarse level");
+    if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
+    mgf_launch_prolong(s->pairs[s->level_pair[igc - 1]], s->st);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
@@ -418,27 +655,31 @@ extern "C" int mgf_prolong(mgf_solver *s, int igc)
 extern "C" int mgf_vcycle(mgf_solver *s, int icyc, double *mass_err)
 {
     if (!s || icyc < 0) return set_err(s, MGF_ERR_ARG, "bad cycle index");
-    int rc = ensure_hist(s, icyc + 1);
+    int rc = single_only(s);
     if (rc) return rc;
-    if ((rc = reset_ctl(s, icyc, 0.0))) return rc;
+    if ((rc = ensure_hist(s, icyc + 1))) return rc;
+    if ((rc = reset_ctl(s, icyc, NULL, NULL))) return rc;
     enqueue_vcycle(s);
     if ((rc = check_launch(s))) return rc;
     double h = 0.0;
-    HIPCHK(s, hipMemcpyAsync(&h, s->hist + icyc, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(s, hipMemcpyAsync(&h, s->d_hist + icyc, sizeof(double), hipMemcpyDeviceToHost, s->st));
     HIPCHK(s, hipStreamSynchronize(s->st));
     if (mass_err) *mass_err = h;
     return MGF_OK;
 }
 
+/* the graph holds no history or control pointers of its own: those live in
+ * the control blocks, so it stays valid when the history buffer grows */
 static int capture_graph(mgf_solver *s)
 {
     if (s->graph_ready) return MGF_OK;
     hipGraph_t g;
     HIPCHK(s, hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
     enqueue_vcycle(s);
-    mgf_launch_end_cycle(s->ctl, s->hist, s->st);
+    mgf_launch_end_cycle(s->d_ctl, s->nb, s->st);
     hipError_t e = hipStreamEndCapture(s->st, &g);
-    if (e != hipSuccess) return set_err(s, MGF_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess)
+        return set_err(s, MGF_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&s->gexec, g, NULL, NULL, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess)
@@ -447,49 +688,115 @@ static int capture_graph(mgf_solver *s)
     return MGF_OK;
 }
 
-extern "C" int mgf_solve(mgf_solver *s, int nvcycle, double tol, double *history, int *ncycles,
-                         int *converged)
+/* Run V-cycles until every instance is done: its recorded residual fell
+ * below its tolerance, or it ran its cycle limit (both decided on the
+ * device).  The host enqueues cycles in growing chunks and polls the control
+ * blocks between chunks; kernels of finished instances return at once, so
+ * cycles enqueued past an instance's end change nothing. */
+static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
 {
-    if (!s || nvcycle < 0) return set_err(s, MGF_ERR_ARG, "bad arguments");
-    int rc = ensure_hist(s, nvcycle > 0 ? nvcycle : 1);
+    int maxnv = 0;
+    for (int b = 0; b < s->nb; ++b) maxnv = limit[b] > maxnv ? limit[b] : maxnv;
+    int rc = ensure_hist(s, maxnv > 0 ? maxnv : 1);
     if (rc) return rc;
-    if ((rc = reset_ctl(s, 0, tol))) return rc;
+    if ((rc = reset_ctl(s, 0, tol, limit))) return rc;
     if (s->opt.use_graph && (rc = capture_graph(s))) return rc;
     int launched = 0, chunk = 1;
-    while (launched < nvcycle) {
-        int k = chunk < nvcycle - launched ? chunk : nvcycle - launched;
+    while (launched < maxnv) {
+        const int k = chunk < maxnv - launched ? chunk : maxnv - launched;
         for (int t = 0; t < k; ++t) {
             if (s->opt.use_graph) {
                 HIPCHK(s, hipGraphLaunch(s->gexec, s->st));
             } else {
                 enqueue_vcycle(s);
-                mgf_launch_end_cycle(s->ctl, s->hist, s->st);
+                mgf_launch_end_cycle(s->d_ctl, s->nb, s->st);
             }
         }
         if ((rc = check_launch(s))) return rc;
         launched += k;
         if ((rc = read_ctl(s))) return rc;
-        if (s->ctl_host->done) break;
+        int all = 1;
+        for (int b = 0; b < s->nb; ++b) all &= s->h_ctl[b].done;
+        if (all) break;
         if (chunk < 32) chunk *= 2;
     }
-    if ((rc = read_ctl(s))) return rc;
-    int nc = s->ctl_host->cyc;
-    if (history && nc > 0)
-        HIPCHK(s, hipMemcpy(history, s->hist, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
-    if (ncycles) *ncycles = nc;
-    if (converged) *converged = s->ctl_host->done;
-    if (s->ctl_host->nonfinite)
-        return set_err(s, MGF_ERR_NONFINITE, "non-finite residual recorded");
+    return read_ctl(s);
+}
+
+static int last_value(mgf_solver *s, int b, double *out)
+{
+    const int nc = s->h_ctl[b].cyc;
+    *out = 0.0;
+    if (nc > 0)
+        HIPCHK(s, hipMemcpy(out, s->d_hist + (size_t)b * s->hist_cap + nc - 1, sizeof(double),
+                            hipMemcpyDeviceToHost));
     return MGF_OK;
+}
+
+extern "C" int mgf_solve(mgf_solver *s, int nvcycle, double tol, double *history, int *ncycles,
+                         int *converged)
+{
+    if (!s || nvcycle < 0) return set_err(s, MGF_ERR_ARG, "bad arguments");
+    int rc = single_only(s);
+    if (rc) return rc;
+    if ((rc = run_cycles(s, &nvcycle, &tol))) return rc;
+    const int nc = s->h_ctl[0].cyc;
+    if (history && nc > 0)
+        HIPCHK(s, hipMemcpy(history, s->d_hist, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
+    if (ncycles) *ncycles = nc;
+    if (converged) {
+        double last;
+        if ((rc = last_value(s, 0, &last))) return rc;
+        *converged = (nc > 0 && last < tol) ? 1 : 0;
+    }
+    if (s->h_ctl[0].nonfinite) return set_err(s, MGF_ERR_NONFINITE, "non-finite residual recorded");
+    return MGF_OK;
+}
+
+extern "C" int mgf_batch_solve(mgf_solver *s, int *ncycles, int *converged)
+{
+    if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
+    std::vector<int> lim(s->nb);
+    std::vector<double> tol(s->nb);
+    for (int b = 0; b < s->nb; ++b) {
+        lim[b] = s->inst[b].nvcycle;
+        tol[b] = s->inst[b].tol;
+    }
+    int rc = run_cycles(s, lim.data(), tol.data());
+    if (rc) return rc;
+    int nonfinite = 0;
+    for (int b = 0; b < s->nb; ++b) {
+        if (ncycles) ncycles[b] = s->h_ctl[b].cyc;
+        if (converged) {
+            double last;
+            if ((rc = last_value(s, b, &last))) return rc;
+            converged[b] = (s->h_ctl[b].cyc > 0 && last < tol[b]) ? 1 : 0;
+        }
+        nonfinite |= s->h_ctl[b].nonfinite;
+    }
+    if (nonfinite) return set_err(s, MGF_ERR_NONFINITE, "non-finite residual recorded");
+    return MGF_OK;
+}
+
+extern "C" int mgf_batch_history(mgf_solver *s, int b, double *history, int cap)
+{
+    if (!s || b < 0 || b >= s->nb) return set_err(s, MGF_ERR_ARG, "bad instance");
+    int rc = read_ctl(s);
+    if (rc) return rc;
+    const int nc = s->h_ctl[b].cyc;
+    const int n = nc < cap ? nc : cap;
+    if (history && n > 0)
+        HIPCHK(s, hipMemcpy(history, s->d_hist + (size_t)b * s->hist_cap,
+                            (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return nc;
 }
 
 /* ---------------------------------------------------------------------- */
 /* data access                                                              */
 /* ---------------------------------------------------------------------- */
 
-static double *field_ptr(mgf_solver *s, int ig, const char *name)
+static double *field_ptr(DevLevel &L, const char *name)
 {
-    DevLevel &L = s->lv[ig];
     struct {
         const char *n;
         double *p;
@@ -501,23 +808,32 @@ static double *field_ptr(mgf_solver *s, int ig, const char *name)
     return NULL;
 }
 
-extern "C" int mgf_get_field(mgf_solver *s, int ig, const char *name, double *out)
+extern "C" int mgf_batch_get_field(mgf_solver *s, int b, int ig, const char *name, double *out)
 {
-    if (!s || ig < 0 || ig >= s->ng || !name || !out) return set_err(s, MGF_ERR_ARG, "bad arguments");
-    double *p = field_ptr(s, ig, name);
+    if (!s || b < 0 || b >= s->nb || ig < 0 || ig >= s->inst[b].ng || !name || !out)
+        return set_err(s, MGF_ERR_ARG, "bad arguments");
+    DevLevel &L = s->inst[b].lv[ig];
+    double *p = field_ptr(L, name);
     if (!p) return set_err(s, MGF_ERR_ARG, "unknown field %s", name);
-    size_t n = (size_t)s->lv[ig].npx * s->lv[ig].npy;
+    const size_t n = (size_t)L.npx * L.npy;
     HIPCHK(s, hipMemcpyAsync(out, p, n * sizeof(double), hipMemcpyDeviceToHost, s->st));
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
 }
 
+extern "C" int mgf_get_field(mgf_solver *s, int ig, const char *name, double *out)
+{
+    return mgf_batch_get_field(s, 0, ig, name, out);
+}
+
 extern "C" int mgf_set_field(mgf_solver *s, int ig, const char *name, const double *in)
 {
-    if (!s || ig < 0 || ig >= s->ng || !name || !in) return set_err(s, MGF_ERR_ARG, "bad arguments");
-    double *p = field_ptr(s, ig, name);
+    if (!s || ig < 0 || ig >= s->inst[0].ng || !name || !in)
+        return set_err(s, MGF_ERR_ARG, "bad arguments");
+    DevLevel &L = s->inst[0].lv[ig];
+    double *p = field_ptr(L, name);
     if (!p) return set_err(s, MGF_ERR_ARG, "unknown field %s", name);
-    size_t n = (size_t)s->lv[ig].npx * s->lv[ig].npy;
+    const size_t n = (size_t)L.npx * L.npy;
     HIPCHK(s, hipMemcpyAsync(p, in, n * sizeof(double), hipMemcpyHostToDevice, s->st));
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
@@ -543,26 +859,32 @@ extern "C" int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, doubl
 extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, double *mean_us,
                                double *bytes_per_launch)
 {
-    if (!s || ig < 0 || ig >= s->ng || reps < 1) return set_err(s, MGF_ERR_ARG, "bad arguments");
+    if (!s || reps < 1) return set_err(s, MGF_ERR_ARG, "bad arguments");
+    int rc = single_only(s);
+    if (rc) return rc;
+    if (ig < 0 || ig >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad level");
     if (kernel != MGF_KERNEL_SWEEP_PP && kernel != MGF_KERNEL_SWEEP_U)
         return set_err(s, MGF_ERR_ARG, "unknown kernel id");
-    DevLevel &L = s->lv[ig];
+    DevLevel &L = s->inst[0].lv[ig];
+    const DevGroup &G = s->groups[s->level_group[ig]];
     const int field = kernel == MGF_KERNEL_SWEEP_PP ? 2 : 0;
     double *phi = field == 2 ? L.pp : L.u;
-    size_t n = (size_t)L.npx * L.npy;
+    const size_t n = (size_t)L.npx * L.npy;
     double *save = NULL;
     HIPCHK(s, hipMalloc(&save, n * sizeof(double)));
     HIPCHK(s, hipMemcpyAsync(save, phi, n * sizeof(double), hipMemcpyDeviceToDevice, s->st));
-    int rc = reset_ctl(s, 0, 0.0);
-    if (rc) { (void)hipFree(save); return rc; }
+    if ((rc = reset_ctl(s, 0, NULL, NULL))) {
+        (void)hipFree(save);
+        return rc;
+    }
     const int isolve = s->isolve == 1 ? 1 : 2;
-    /* warm up, then time */
-    mgf_launch_smooth(L, field, isolve, 1, s->ctl, s->st);
+    const int even = reps + (reps & 1);    /* an even count ends without a copy-back */
+    mgf_launch_smooth(G, field, isolve, 2, s->st);        /* warm up */
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s->st);
-    mgf_launch_smooth(L, field, isolve, reps, s->ctl, s->st);
+    mgf_launch_smooth(G, field, isolve, even, s->st);
     (void)hipEventRecord(e1, s->st);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -573,11 +895,10 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     (void)hipStreamSynchronize(s->st);
     (void)hipFree(save);
     if ((rc = check_launch(s))) return rc;
-    int nx = field == 2 ? L.npx - 1 : L.npx - 2, ny = L.npy - 1;
-    double nunk = (double)(nx - 1) * (double)(ny - 1);
-    /* one launch per sweep (Jacobi sweep, or both red-black colours) */
-    int per_sweep = 1;
-    if (mean_us) *mean_us = 1000.0 * (double)ms / ((double)reps * per_sweep);
-    if (bytes_per_launch) *bytes_per_launch = 8.0 * 8.0 * nunk / per_sweep;
+    const int nx = field == 2 ? L.npx - 1 : L.npx - 2, ny = L.npy - 1;
+    const double nunk = (double)(nx - 1) * (double)(ny - 1);
+    /* one launch per sweep (a Jacobi sweep, or both red-black colours) */
+    if (mean_us) *mean_us = 1000.0 * (double)ms / (double)even;
+    if (bytes_per_launch) *bytes_per_launch = 8.0 * 8.0 * nunk;
     return MGF_OK;
 }

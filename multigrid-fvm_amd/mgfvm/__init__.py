@@ -6,7 +6,7 @@ holds the ctypes binding (`Solver`) and the input-file tools (`inputs`).
 """
 from . import inputs
 from ._lib import (EXPORTS, KERNEL_SWEEP_PP, KERNEL_SWEEP_U, LIB_PATH, PREC_FP64,
-                   PREC_MIXED, MgfError, Solver, lib)
+                   PREC_MIXED, Batch, MgfError, Solver, lib)
 
-__all__ = ["inputs", "Solver", "MgfError", "lib", "LIB_PATH", "EXPORTS", "PREC_FP64",
+__all__ = ["inputs", "Solver", "Batch", "MgfError", "lib", "LIB_PATH", "EXPORTS", "PREC_FP64",
            "PREC_MIXED", "KERNEL_SWEEP_PP", "KERNEL_SWEEP_U"]

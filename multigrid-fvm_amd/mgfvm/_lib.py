@@ -21,7 +21,9 @@ EXPORTS = [
     "mgf_abi_version", "mgf_create", "mgf_destroy", "mgf_last_error", "mgf_num_levels",
     "mgf_level_dims", "mgf_stream", "mgf_relax", "mgf_restrict", "mgf_prolong",
     "mgf_vcycle", "mgf_solve", "mgf_get_field", "mgf_set_field", "mgf_get_fields",
-    "mgf_time_kernel", "mgf_reset", "mgf_synchronize",
+    "mgf_time_kernel", "mgf_reset", "mgf_synchronize", "mgf_batch_create", "mgf_batch_size",
+    "mgf_batch_num_levels", "mgf_batch_level_dims", "mgf_batch_solve", "mgf_batch_history",
+    "mgf_batch_get_field",
 ]
 
 PREC_FP64 = 0
@@ -95,6 +97,15 @@ def lib():
         L.mgf_time_kernel.argtypes = [vp, ip, ip, ip, P_d, P_d]
         L.mgf_reset.argtypes = [vp]
         L.mgf_synchronize.argtypes = [vp]
+        PP = ctypes.POINTER(ProblemStruct)
+        L.mgf_batch_create.argtypes = [ctypes.POINTER(PP), ip, ctypes.POINTER(Options),
+                                       ctypes.POINTER(vp)]
+        L.mgf_batch_size.argtypes = [vp]
+        L.mgf_batch_num_levels.argtypes = [vp, ip]
+        L.mgf_batch_level_dims.argtypes = [vp, ip, ip, P_i, P_i]
+        L.mgf_batch_solve.argtypes = [vp, P_i, P_i]
+        L.mgf_batch_history.argtypes = [vp, ip, P_d, ip]
+        L.mgf_batch_get_field.argtypes = [vp, ip, ip, cp, P_d]
         _lib = L
     return _lib
 
@@ -223,3 +234,79 @@ class Solver:
         us, b = ctypes.c_double(), ctypes.c_double()
         self._check(lib().mgf_time_kernel(self.h, kernel, ig, reps, ctypes.byref(us), ctypes.byref(b)))
         return us.value, b.value
+
+
+class Batch:
+    """Many independent problems solved together on one GPU."""
+
+    def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
+                 use_graph: bool = True, device: int = -1):
+        L = lib()
+        probs = [read_problem(a, b) for a, b in pairs]
+        try:
+            arr = (ctypes.POINTER(ProblemStruct) * len(probs))(*probs)
+            opt = Options()
+            L.mgf_options_default(ctypes.byref(opt))
+            opt.precision = precision
+            opt.use_graph = 1 if use_graph else 0
+            opt.device = device
+            h = ctypes.c_void_p()
+            rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))
+            if rc != 0:
+                raise MgfError(rc, L.mgf_error_string().decode())
+            self.h = h
+            self.n = len(probs)
+        finally:
+            for p in probs:
+                free_problem(p)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise MgfError(rc, lib().mgf_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        self._check(lib().mgf_reset(self.h))
+
+    def synchronize(self):
+        self._check(lib().mgf_synchronize(self.h))
+
+    def solve(self):
+        nc = (ctypes.c_int * self.n)()
+        cv = (ctypes.c_int * self.n)()
+        rc = lib().mgf_batch_solve(self.h, nc, cv)
+        if rc not in (0, -8):
+            self._check(rc)
+        return list(nc), [bool(x) for x in cv]
+
+    def history(self, b: int) -> np.ndarray:
+        cap = 1 << 16
+        out = np.zeros(cap, np.float64)
+        n = lib().mgf_batch_history(self.h, b, _dptr(out), cap)
+        if n < 0:
+            self._check(n)
+        return out[:n].copy()
+
+    def num_levels(self, b: int) -> int:
+        return lib().mgf_batch_num_levels(self.h, b)
+
+    def dims(self, b: int, ig: int) -> Tuple[int, int]:
+        a, c = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().mgf_batch_level_dims(self.h, b, ig, ctypes.byref(a), ctypes.byref(c)))
+        return a.value, c.value
+
+    def get(self, b: int, ig: int, name: str) -> np.ndarray:
+        nx, ny = self.dims(b, ig)
+        out = np.empty(nx * ny, np.float64)
+        self._check(lib().mgf_batch_get_field(self.h, b, ig, name.encode(), _dptr(out)))
+        return out.reshape(ny, nx)
