@@ -43,7 +43,7 @@ struct DevLevel {
     int has_obs;
     int ext;              /* level below the obstacle level: coarsened obstacles */
     int nblk;             /* workgroups of the residual kernel (partials) */
-    int pad;
+    int isolve;           /* 1 Jacobi, 2 red-black */
     /* metrics, each computed on the host with the reference's expression */
     double ax, ay;        /* areax (= dy), areay (= dx) */
     double du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, wall adds */
@@ -62,6 +62,10 @@ struct DevLevel {
     double *tmp;                     /* smoother ping-pong */
     double *cu, *cv, *cp, *csc;      /* coarse-grid corrections (ghosts stay 0) */
     double *partial;                 /* per-block partial sums of |div| */
+    /* mixed precision (null in FP64 mode): FP32 coefficients, 1/diagonal,
+     * right-hand side (the FP64 residual for u, v, sc; -div for p'), and
+     * the two buffers of the FP32 correction sweeps */
+    float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32;
     /* obstacle data */
     const uint8_t *flags;
     const uint64_t *sccode;
@@ -100,6 +104,9 @@ void mgf_launch_outflow(const DevGroup &G, hipStream_t st);
 void mgf_launch_restrict(const DevPair &R, hipStream_t st);
 void mgf_launch_prolong(const DevPair &R, hipStream_t st);
 void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
+/* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
+void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st);
+void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
 void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st);
 int mgf_partial_blocks(int npx, int npy);
 

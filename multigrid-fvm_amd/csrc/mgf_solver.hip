@@ -109,10 +109,11 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 /* arena                                                                    */
 /* ---------------------------------------------------------------------- */
 
-static size_t level_bytes(const mgf_plan_level *l)
+static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
     size_t t = 27 * align256(n * sizeof(double));  /* 23 arrays + 4 initial-state copies */
+    if (mixed) t += 8 * align256(n * sizeof(float));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
     t += align256(n * sizeof(uint64_t));            /* sccode */
@@ -170,6 +171,12 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl)
         for (int k = 0; k < 23; ++k) *arrs[k] = (double *)take(nb);
         I.init_state[ig] = (double *)take(4 * nb);
         L.partial = (double *)take((size_t)L.nblk * sizeof(double));
+        L.isolve = c->isolve;
+        if (s->opt.precision == MGF_PREC_MIXED) {
+            float **a32[8] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
+                              &L.da32, &L.db32};
+            for (int k = 0; k < 8; ++k) *a32[k] = (float *)take(n * sizeof(float));
+        }
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
         L.pslot = (const int *)take(n * sizeof(int));
@@ -362,7 +369,7 @@ extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mg
         o = *opt;
     else
         mgf_options_default(&o);
-    if (o.precision != MGF_PREC_FP64)
+    if (o.precision != MGF_PREC_FP64 && o.precision != MGF_PREC_MIXED)
         return set_err(NULL, MGF_ERR_UNSUPPORTED, "precision mode %d not available", o.precision);
     mgf_solver *s = new mgf_solver();
     s->opt = o;
@@ -421,7 +428,8 @@ extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mg
     }
     size_t total = 0;
     for (Inst &I : s->inst)
-        for (int ig = 0; ig < I.ng; ++ig) total += level_bytes(&I.plan->lv[ig]);
+        for (int ig = 0; ig < I.ng; ++ig)
+            total += level_bytes(&I.plan->lv[ig], o.precision == MGF_PREC_MIXED);
     s->arena_bytes = total;
     if (hipMalloc(&s->arena, total) != hipSuccess) {
         s->arena = nullptr;
@@ -509,8 +517,37 @@ extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
 /* ---------------------------------------------------------------------- */
 
 /* V6:201-207 (leg 0) and V6:237-243 (leg 1) for a group */
+/* mixed precision: the same steps with FP64 assembly + residual, FP32
+ * correction sweeps and FP64 apply (mgf_kernels_mixed.hip) */
+static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg)
+{
+    hipStream_t st = s->st;
+    mgf_launch_fluxes(G, st);
+    mgf_launch_asm_mixed(G, 0, 0, st);
+    mgf_launch_smooth_mixed(G, 0, s->isolve, s->nswpm, st);
+    mgf_launch_asm_mixed(G, 1, 0, st);
+    mgf_launch_smooth_mixed(G, 1, s->isolve, s->nswpm, st);
+    mgf_launch_asm_mixed(G, 2, leg == 0, st);
+    mgf_launch_smooth_mixed(G, 2, s->isolve, s->nswpp, st);
+    mgf_launch_update(G, st);
+    mgf_launch_pbound(G, st);
+    if (leg == 0) {
+        mgf_launch_outflow(G, st);
+        mgf_launch_asm_mixed(G, 3, 0, st);
+        mgf_launch_smooth_mixed(G, 3, s->isolve, s->nswpm, st);
+    } else {
+        mgf_launch_asm_mixed(G, 3, 0, st);
+        mgf_launch_smooth_mixed(G, 3, s->isolve, s->nswpm, st);
+        mgf_launch_outflow(G, st);
+    }
+}
+
 static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg)
 {
+    if (s->opt.precision == MGF_PREC_MIXED) {
+        enqueue_relax_mixed(s, G, leg);
+        return;
+    }
     hipStream_t st = s->st;
     mgf_launch_fluxes(G, st);
     mgf_launch_asm_u(G, st);

@@ -287,3 +287,24 @@ def parse(inputdata: str, pattern: str) -> Problem:
     n = ni()
     p.outlets = [(ni(), ni(), nf(), nf(), nf()) for _ in range(max(n, 0))]
     return p
+
+
+def staggered_batch(count: int = 256, seed0: int = 1000):
+    """Many channel problems for one batched solve: grid n drawn uniformly
+    from {128, 256, 512} with the level count that makes the coarsest level
+    16x16, a staggered obstacle array (spacing 4 base cells on a 16x16 base)
+    with every obstacle moved by a quarter, half or full base cell, and
+    Re drawn from {10, 50, 100, 200, 400}; red-black 2/6, tolerance 1e-6.
+    Instance k uses seed seed0 + k for every draw."""
+    out = []
+    for k in range(count):
+        rng = Rng(seed0 + k)
+        n = (128, 256, 512)[rng.below(3)]
+        shift = ("quarter", "half", "full")[rng.below(3)]
+        re = (10, 50, 100, 200, 400)[rng.below(5)]
+        rects, scale = staggered_pattern(16, 4, shift, seed0 + k)
+        ngrid = int(round(math.log2(n))) - 3
+        ig_obs = ngrid - int(round(math.log2(scale)))
+        out.append(Problem(nfx=n, nfy=n, ngrid=ngrid, ig_obs=ig_obs, nvcycle=2999, nswpm=2,
+                           nswpp=6, isolve=2, amuf=1.0 / re, obstacles=rects))
+    return out

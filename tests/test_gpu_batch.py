@@ -88,3 +88,51 @@ def test_batch_rejects_mixed_smoothers(tmp_path):
     pairs = [p.write(str(tmp_path / "a")), q.write(str(tmp_path / "b"))]
     with pytest.raises(mgfvm.MgfError):
         mgfvm.Batch(pairs)
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (FP32 correction sweeps, FP64 residuals and state)
+# ---------------------------------------------------------------------------
+MIXED_FIELD_RTOL = 1e-6     # max |mixed - fp64| / max |fp64| per field, measured <= 1.1e-7
+
+
+@pytest.mark.parametrize("name", ["cfg0_rb", "rand64", "stag128", "coarse_free", "small_jac"])
+def test_mixed_precision_tracks_fp64(tmp_path, name):
+    """Mixed precision keeps the FP64 stopping test: same cycle count and
+    converged flag as the FP64 oracle, history and every level-0 field within
+    MIXED_FIELD_RTOL (relative to the field's magnitude)."""
+    import cases
+    import mgfvm
+    pin, ppat = cases.write_case(name, str(tmp_path))
+    p = cases.CASES[name]
+    _say(f"[gpu] mixed {name}")
+    m = mgfvm.Solver(pin, ppat, precision=mgfvm.PREC_MIXED)
+    hm, nm, cm = m.solve()
+    o = orc.Oracle(pin, ppat)
+    ho, no, co = o.solve(p.nvcycle, p.convergence)
+    assert (nm, cm) == (no, co)
+    np.testing.assert_allclose(hm, ho, rtol=1e-5, atol=1e-12)
+    for n in ("u", "v", "p", "pp", "sc"):
+        g, r = m.get(0, n), o.get(0, n)
+        scale = max(np.abs(r).max(), 1e-300)
+        assert np.abs(g - r).max() / scale < MIXED_FIELD_RTOL, n
+    o.close()
+
+
+def test_mixed_batch_matches_mixed_singles(tmp_path):
+    """Batching does not change mixed-precision results: every member is
+    bit-identical to its own single mixed solve."""
+    import mgfvm
+    probs = _mixed_problems()[:4]
+    pairs = [p.write(str(tmp_path / f"m{k}")) for k, p in enumerate(probs)]
+    _say("[gpu] mixed batch vs singles")
+    bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED)
+    ncyc, conv = bt.solve()
+    for k, (pin, ppat) in enumerate(pairs):
+        s = mgfvm.Solver(pin, ppat, precision=mgfvm.PREC_MIXED)
+        h, n, c = s.solve()
+        assert (n, c) == (ncyc[k], conv[k])
+        assert np.array_equal(h, bt.history(k))
+        for name in ("u", "v", "p", "sc"):
+            assert np.array_equal(s.get(0, name), bt.get(k, 0, name)), (k, name)
+        s.close()
