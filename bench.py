@@ -1,26 +1,31 @@
 #!/usr/bin/env python3
 """Converged-solve throughput of the MI355X SIMPLE/FAS-multigrid solver.
 
-A step is one complete steady solve from the initial state to the stopping
-test (level-0 sum |div| < 1e-6, FP64), inputs already resident in HBM: the
-solver is created once, and every step restores its initial state on the
-device (mgf_reset) and runs graph-replayed V-cycles until the device-side
-stopping test fires (mgf_solve).
+Metric: converged solves per second at the FP64 stopping test (level-0 sum
+of |div| < 1e-6, evaluated in FP64), batched mixed precision vs the plain
+FP64 GPU path, with the dominant kernel's fraction of HBM peak.
 
-Default workload: 1024x1024 channel, random obstacle mask on a 64x64 base
-grid (Bernoulli, porosity 0.8, seed 1; each blocked base cell is 16x16 fine
-cells), Re 200, red-black Gauss-Seidel 2/6 sweeps, 5 multigrid levels
-(coarsest 64x64, the obstacle level).  With more levels the coarse grids
-cannot represent the obstacles and the reference's algorithm diverges
-(7 levels) -- see DESIGN.md.
+A step is one batched solve of a fixed set of 256 independent channel
+problems (grids 128^2/256^2/512^2 with 16x16 coarsest level, staggered
+obstacle arrays moved by quarter/half/full cells, Re in {10,50,100,200,400},
+red-black 2/6; seeds 1000..1255), inputs already resident in HBM: the batch
+is created once, each step restores the initial state on the device
+(mgf_reset) and runs graph-replayed batched V-cycles until every problem has
+met its stopping test or its 2999-cycle limit (mgf_batch_solve).  `value`
+counts the problems that converged, per second.  Problems the reference's
+algorithm does not converge (low Re on 512^2 with a 16^2 coarsest level,
+identical in the CPU restatement) run their full cycle limit and count as
+not converged.
 
-Printed: one JSON line with the throughput, the roofline of the dominant
-kernel (one red-black sweep of the p' equation on level 0, both colours in
-one launch, HIP events on the solver's stream) and a CPU baseline (the reference program built from
-its own source, single core, a bounded number of V-cycles of the same
-solve, scaled to the cycles the solve needs).
+Also reported: the plain FP64 GPU path (one problem at a time, same
+problems, a sample of --plain-sample problems) and the ratio; a single
+1024^2 solve (64x64 random mask, Re 200, 5 levels) in FP64 and mixed; the
+roofline of the dominant kernel (one FP32 red-black sweep launch over the
+512^2 members, HIP events on the solver's stream); and a CPU baseline (the
+reference program built from its own source, one core, per-V-cycle time per
+grid size measured on bounded runs, times the cycle counts of the batch).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
@@ -38,132 +43,200 @@ sys.path.insert(0, os.path.join(ROOT, "multigrid-fvm_amd"))
 METRIC = ("converged solves/s at FP64 residual 1e-6, batched-mixed vs plain GPU; "
           "% HBM peak")
 HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
+            "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
+            "each obstacle moved by a seeded quarter/half/full cell), Re in "
+            "{10,50,100,200,400}, red-black 2/6, FP32 correction sweeps with FP64 "
+            "residuals and state, stop at FP64 sum|div| < 1e-6 or 2999 V-cycles")
 
 
-def workloads():
-    from mgfvm.inputs import channel_128_inline, random_1024
-    return {
-        "rand1024_l5": (random_1024(ngrid=5),
-                        "1024x1024 channel, 64x64 Bernoulli(porosity 0.8, seed 1) obstacle mask, "
-                        "Re 200, red-black 2/6, 5 levels (coarsest = obstacle level 64x64)"),
-        "rand1024_l6": (random_1024(ngrid=6),
-                        "1024x1024 channel, 64x64 Bernoulli(porosity 0.8, seed 1) obstacle mask, "
-                        "Re 200, red-black 2/6, 6 levels (one level below the obstacle level)"),
-        "inline128_rb": (channel_128_inline().replace(isolve=2),
-                         "128x128 channel, inline 4x4 single-cell blocks on 16x16, Re 100, "
-                         "red-black 2/4, 4 levels"),
-    }
+def _sync(obj):
+    import torch
+    obj.synchronize()
+    torch.cuda.synchronize()
 
 
-def cpu_baseline(prob, ncycles_full, workdir, target_s=12.0):
-    """Time the reference program (oracle/_ref/ref_plain, built from the
-    reference's own source) or, where it was not built, the CPU restatement
-    (oracle/orc_run), single core, on the same inputs for a bounded number of
-    V-cycles; scale per-cycle time to the cycles the full solve takes."""
+def time_reference_cycle(prob, workdir, target_s=4.0):
+    """Seconds per V-cycle of the reference program (oracle/_ref/ref_plain,
+    built from the reference's own source) -- or of the CPU restatement
+    (oracle/orc_run) where that was not built -- on one core, measured on a
+    bounded run of the problem (V-cycle loop time only)."""
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_plain")
     port = os.path.join(ROOT, "oracle", "orc_run")
     kind = "reference" if os.path.exists(ref) else "port"
     exe = ref if kind == "reference" else port
     if not os.path.exists(exe):
-        return None
+        return None, None, 0
+    taskset = shutil.which("taskset")
+    pre = [taskset, "-c", "0"] if taskset else []
 
     def run(ncyc):
-        d = os.path.join(workdir, f"cpu_{ncyc}")
+        d = os.path.join(workdir, f"cpu_{prob.nfx}_{ncyc}")
         shutil.rmtree(d, ignore_errors=True)
         prob.replace(nvcycle=ncyc, convergence=0.0).write(d)
-        taskset = shutil.which("taskset")
-        pre = [taskset, "-c", "0"] if taskset else []
         t0 = time.perf_counter()
         if kind == "reference":
             r = subprocess.run(pre + [exe], cwd=d, capture_output=True, text=True)
-            wall = time.perf_counter() - t0
             m = re.search(r"simple solution took ([0-9.eE+-]+) seconds", r.stdout)
-            secs = float(m.group(1)) if m else wall
         else:
             r = subprocess.run(pre + [exe, "InputData", "pattern.txt", os.path.join(d, "o.bin")],
                                cwd=d, capture_output=True, text=True)
             m = re.search(r"seconds ([0-9.eE+-]+)", r.stderr)
-            secs = float(m.group(1)) if m else time.perf_counter() - t0
-        return secs
+        return float(m.group(1)) if m else time.perf_counter() - t0
 
     probe = 2
-    t_probe = run(probe)
-    per = t_probe / probe
-    n = max(probe, min(ncycles_full, int(target_s / max(per, 1e-6))))
-    t = run(n) if n != probe else t_probe
-    per_cycle = t / n
-    solve_s = per_cycle * ncycles_full
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": kind,
-            "sample": f"{n} of the {ncycles_full} V-cycles of the same solve on 1 core "
-                      f"({per_cycle * 1e3:.1f} ms/V-cycle, V-cycle loop time only); "
-                      f"solve time = per-cycle time x {ncycles_full}"}
+    per = run(probe) / probe
+    n = max(probe, int(target_s / max(per, 1e-6)))
+    return (run(n) / n if n != probe else per), kind, n
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rand1024_l5")
+    ap.add_argument("--count", type=int, default=256, help="problems in the batch")
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="problems solved one at a time for the plain FP64 GPU figure")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cycles", type=int, default=0,
-                    help="cap V-cycles per solve (profiling runs only; the line is then not "
-                         "a converged-solve figure)")
+    ap.add_argument("--no-single", action="store_true", help="skip the 1024^2 single solve")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-at-a-time figure")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(instances are independent and shard by instance; see DESIGN.md)",
+              "(problems are independent and would shard by problem; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
 
     import torch
     import mgfvm
+    from mgfvm.inputs import random_1024, staggered_batch
 
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
-    prob, desc = workloads()[args.workload]
     work = tempfile.mkdtemp(prefix="mgf_bench_")
     try:
-        pin, ppat = prob.write(os.path.join(work, "case"))
-        s = mgfvm.Solver(pin, ppat, device=0)
-        nv, tol = prob.nvcycle, prob.convergence
-        if args.cycles > 0:
-            nv, tol = args.cycles, 0.0
-        ncyc = conv = None
+        probs = staggered_batch(args.count)
+        pairs = [p.write(os.path.join(work, f"p{k}")) for k, p in enumerate(probs)]
+
+        # ---- batched mixed precision (the measured line) ----------------
+        bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, device=0)
         for _ in range(args.warmup):
-            s.reset()
-            _, ncyc, conv = s.solve(nv, tol)
-        torch.cuda.synchronize()
-        s.synchronize()
+            bt.reset()
+            bt.solve()
+        _sync(bt)
         t0 = time.perf_counter()
+        conv_total = 0
         for _ in range(args.steps):
-            s.reset()
-            hist, ncyc, conv = s.solve(nv, tol)
-        s.synchronize()
-        torch.cuda.synchronize()
+            bt.reset()
+            ncyc, conv = bt.solve()
+            conv_total += sum(conv)
+        _sync(bt)
         elapsed = time.perf_counter() - t0
-        if not conv:
-            print(f"bench.py: workload did not converge in {nv} V-cycles", file=sys.stderr)
-        # dominant kernel: one red-black sweep (both colours) of the p' equation on level 0
-        us, nbytes = s.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, 200)
+        value = conv_total / elapsed
+        bt.close()
+        print(f"[bench] batched mixed: {conv_total} converged in {elapsed:.2f}s", file=sys.stderr,
+              flush=True)
+
+        # ---- roofline: one FP32 sweep launch over the largest members ----
+        nmax = max(p.nfx for p in probs)
+        big_pairs = [pairs[k] for k in range(len(probs)) if probs[k].nfx == nmax]
+        bb = mgfvm.Batch(big_pairs, precision=mgfvm.PREC_MIXED, device=0)
+        bb.reset()
+        bb.solve()          # leaves assembled coefficients behind
+        us, nbytes = bb.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, 100)
+        bb.close()
         achieved = nbytes / (us * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_rb_sweep (one red-black sweep of the p' equation, level 0)",
-                "mean_us": round(us, 3), "bytes_per_launch": nbytes}
-        s.close()
-        cpu = None if args.no_cpu else cpu_baseline(prob, ncyc, work)
-        value = args.steps / elapsed
+                "kernel": f"k_rb32: one FP32 red-black sweep of the p' equation over the "
+                          f"{len(big_pairs)} {nmax}^2 members (one launch)",
+                "mean_us": round(us, 2), "bytes_per_launch": nbytes}
+
+        # ---- plain FP64 GPU path: one problem at a time -----------------
+        plain_value, ks = None, []
+        if not args.no_plain:
+            ks = list(range(min(args.plain_sample, len(probs))))
+            plain_t, plain_conv = 0.0, 0
+            for k in ks:
+                s = mgfvm.Solver(*pairs[k], device=0)
+                s.solve()
+                s.reset()
+                _sync(s)
+                t0 = time.perf_counter()
+                _, n, c = s.solve()
+                _sync(s)
+                plain_t += time.perf_counter() - t0
+                plain_conv += int(c)
+                s.close()
+            plain_value = plain_conv / plain_t if plain_t > 0 else None
+            print(f"[bench] plain fp64: {plain_conv} converged in {plain_t:.2f}s", file=sys.stderr,
+                  flush=True)
+
+        # ---- single large problem ----------------------------------------
+        single = None
+        if not args.no_single:
+            p1 = random_1024(ngrid=5)
+            pin, ppat = p1.write(os.path.join(work, "r1024"))
+            single = {"workload": "1024x1024 channel, 64x64 Bernoulli(porosity 0.8, seed 1) "
+                                  "mask, Re 200, red-black 2/6, 5 levels"}
+            for prec, tag in ((mgfvm.PREC_FP64, "fp64"), (mgfvm.PREC_MIXED, "mixed")):
+                s = mgfvm.Solver(pin, ppat, precision=prec, device=0)
+                s.solve()
+                s.reset()
+                _sync(s)
+                t0 = time.perf_counter()
+                _, n, c = s.solve()
+                _sync(s)
+                dt = time.perf_counter() - t0
+                single[f"{tag}_solves_per_s"] = round(1.0 / dt, 4) if c else 0.0
+                single["v_cycles"] = n
+                if tag == "fp64":
+                    us1, b1 = s.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, 100)
+                    single["fp64_sweep_GBps"] = round(b1 / (us1 * 1e-6) / 1e9, 1)
+                s.close()
+
+        # ---- CPU baseline ---------------------------------------------------
+        cpu = None
+        if not args.no_cpu:
+            per_cycle, kinds, samples = {}, set(), []
+            for n in sorted({p.nfx for p in probs}):
+                # a problem of that size whose obstacles sit on the coarsest
+                # level (so the reference program itself can run it)
+                cand = [p for p in probs if p.nfx == n and p.ig_obs == p.ngrid] or \
+                       [p for p in probs if p.nfx == n]
+                t, kind, ncyc_s = time_reference_cycle(cand[0], work)
+                if t is None:
+                    continue
+                per_cycle[n] = t
+                kinds.add(kind)
+                samples.append(f"{n}^2: {ncyc_s} V-cycles at {t * 1e3:.1f} ms/V-cycle")
+            if len(per_cycle) == len({p.nfx for p in probs}):
+                total = sum(per_cycle[p.nfx] * c for p, c in zip(probs, ncyc))
+                cpu = {"value": sum(conv) / total, "unit": "solves/s", "cores": 1,
+                       "kind": "reference" if kinds == {"reference"} else "port",
+                       "sample": "one core; per-V-cycle time of each grid size from a bounded "
+                                 "run of one problem of that size (" + "; ".join(samples) +
+                                 "), times the V-cycles each problem of the batch needs; "
+                                 f"{sum(conv)} of {len(probs)} problems converge"}
+
         line = {
-            "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
+            "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": desc, "name": args.workload, "v_cycles_per_solve": ncyc,
-                       "converged": bool(conv), "precision": "fp64 (bit-exact)",
-                       "ms_per_v_cycle": round(1e3 * elapsed / args.steps / max(ncyc, 1), 4)},
+            "ms_per_step": round(1e3 * elapsed / args.steps, 2), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": WORKLOAD, "problems": len(probs),
+                       "converged_per_step": sum(conv),
+                       "v_cycles_mean": round(sum(ncyc) / len(ncyc), 1),
+                       "v_cycles_max": max(ncyc),
+                       "plain_fp64_gpu_solves_per_s":
+                           round(plain_value, 3) if plain_value else None,
+                       "plain_sample_problems": len(ks),
+                       "batched_mixed_vs_plain":
+                           round(value / plain_value, 2) if plain_value else None,
+                       "single_1024": single},
             "roofline": roof, "cpu_baseline": cpu,
         }
         print(json.dumps(line))

@@ -190,10 +190,14 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
 /* Kernel ids for mgf_time_kernel */
 #define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation */
 #define MGF_KERNEL_SWEEP_U   1   /* one full smoother sweep of the u equation */
-/* Time `reps` launches of one kernel of level ig with HIP events on the
- * solver's stream, on the current state (the state is saved and restored).
- * Returns the mean duration in microseconds and the algorithmic bytes one
- * launch moves (8 words x word size x unknowns). */
+/* Time `reps` launches of one smoother sweep of level ig (for a batch: the
+ * launch covering every problem's level at the depth of problem 0's level
+ * ig) with HIP events on the solver's stream, on the current coefficients.
+ * FP64 mode works in place on a single problem and restores the field; mixed
+ * mode sweeps only its correction buffers.  Returns the mean duration of one
+ * launch in microseconds and the algorithmic bytes it moves (8 words per
+ * unknown: 4 neighbour coefficients, diagonal, right-hand side, the unknown
+ * read and written; 8 bytes each in FP64, 4 in FP32). */
 int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps,
                     double *mean_us, double *bytes_per_launch);
 

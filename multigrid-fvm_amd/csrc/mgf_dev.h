@@ -81,6 +81,8 @@ struct DevGroup {
     int npx, npy;
     int max_npcopy;
     int any_pbound_x, any_pbound_y, any_outflow;
+    int depth;               /* levels above the coarsest */
+    int up;                  /* 1: the up-leg subset (no finest levels) */
 };
 
 /* a restriction / prolongation between two same-length groups */
@@ -107,6 +109,8 @@ void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
 void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st);
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
+/* the FP32 sweeps alone (they only touch the correction buffers) */
+void mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
 void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st);
 int mgf_partial_blocks(int npx, int npy);
 

@@ -287,6 +287,25 @@ void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t 
     hipLaunchKernelGGL(k_asm_mx, grid3(ni, nj, G.count), dim3(BX, BY), 0, st, G.d, field);
 }
 
+void mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
+{
+    int nx, ny;
+    switch (field) {
+    case 0: nx = G.npx - 2; ny = G.npy - 1; break;
+    case 1: nx = G.npx - 1; ny = G.npy - 2; break;
+    default: nx = G.npx - 1; ny = G.npy - 1; break;
+    }
+    if (nx - 1 < 1 || ny - 1 < 1 || (isolve != 1 && isolve != 2)) return;
+    for (int it = 0; it < nsw; ++it) {
+        if (isolve == 2)
+            hipLaunchKernelGGL(k_rb32, grid3(nx - 1, ny - 1, G.count, RB_TY), dim3(BX, BY), 0, st,
+                               G.d, field, it);
+        else
+            hipLaunchKernelGGL(k_jac32, grid3(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
+                               field, it);
+    }
+}
+
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
 {
     int nx, ny;

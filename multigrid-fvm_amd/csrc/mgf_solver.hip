@@ -304,6 +304,10 @@ static int build_program(mgf_solver *s)
     host.reserve(ndesc);
     s->groups.assign(gm.size(), DevGroup());
     s->pairs.assign(pf.size(), DevPair());
+    for (const auto &e : gkey) {
+        s->groups[e.second].depth = e.first.d;
+        s->groups[e.second].up = e.first.kind;
+    }
     for (size_t g = 0; g < gm.size(); ++g) {
         DevGroup &G = s->groups[g];
         G.d = s->d_desc + host.size();
@@ -897,45 +901,67 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
                                double *bytes_per_launch)
 {
     if (!s || reps < 1) return set_err(s, MGF_ERR_ARG, "bad arguments");
-    int rc = single_only(s);
-    if (rc) return rc;
     if (ig < 0 || ig >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad level");
     if (kernel != MGF_KERNEL_SWEEP_PP && kernel != MGF_KERNEL_SWEEP_U)
         return set_err(s, MGF_ERR_ARG, "unknown kernel id");
-    DevLevel &L = s->inst[0].lv[ig];
-    const DevGroup &G = s->groups[s->level_group[ig]];
-    const int field = kernel == MGF_KERNEL_SWEEP_PP ? 2 : 0;
-    double *phi = field == 2 ? L.pp : L.u;
-    const size_t n = (size_t)L.npx * L.npy;
-    double *save = NULL;
-    HIPCHK(s, hipMalloc(&save, n * sizeof(double)));
-    HIPCHK(s, hipMemcpyAsync(save, phi, n * sizeof(double), hipMemcpyDeviceToDevice, s->st));
-    if ((rc = reset_ctl(s, 0, NULL, NULL))) {
-        (void)hipFree(save);
-        return rc;
+    const int mixed = s->opt.precision == MGF_PREC_MIXED;
+    int rc;
+    if (!mixed && (rc = single_only(s))) return rc;
+    /* the group of every instance's level at instance 0's depth of level ig */
+    const int d = s->inst[0].ng - 1 - ig;
+    const DevLevel &L0 = s->inst[0].lv[ig];
+    int g = -1;
+    for (size_t k = 0; k < s->groups.size(); ++k) {
+        const DevGroup &G = s->groups[k];
+        if (!G.up && G.depth == d && G.npx == L0.npx && G.npy == L0.npy) g = (int)k;
     }
+    if (g < 0) return set_err(s, MGF_ERR_ARG, "no group for level %d", ig);
+    const DevGroup &G = s->groups[g];
+    const int field = kernel == MGF_KERNEL_SWEEP_PP ? 2 : 0;
     const int isolve = s->isolve == 1 ? 1 : 2;
     const int even = reps + (reps & 1);    /* an even count ends without a copy-back */
-    mgf_launch_smooth(G, field, isolve, 2, s->st);        /* warm up */
+    double *save = NULL, *phi = NULL;
+    const size_t n = (size_t)L0.npx * L0.npy;
+    if (!mixed) {
+        /* FP64 sweeps work in place: keep the field */
+        phi = field == 2 ? L0.pp : L0.u;
+        HIPCHK(s, hipMalloc(&save, n * sizeof(double)));
+        HIPCHK(s, hipMemcpyAsync(save, phi, n * sizeof(double), hipMemcpyDeviceToDevice, s->st));
+    }
+    /* the sweeps check the stop flags: clear them for the measurement */
+    for (int b = 0; b < s->nb; ++b) s->h_ctl[b].done = 0;
+    (void)hipMemcpyAsync(s->d_ctl, s->h_ctl, (size_t)s->nb * sizeof(DevCtl), hipMemcpyHostToDevice,
+                         s->st);
+    auto launch = [&](int nsw) {
+        if (mixed)
+            mgf_launch_sweeps_mixed(G, field, isolve, nsw, s->st);
+        else
+            mgf_launch_smooth(G, field, isolve, nsw, s->st);
+    };
+    launch(2);                                            /* warm up */
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s->st);
-    mgf_launch_smooth(G, field, isolve, even, s->st);
+    launch(even);
     (void)hipEventRecord(e1, s->st);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipMemcpyAsync(phi, save, n * sizeof(double), hipMemcpyDeviceToDevice, s->st);
-    (void)hipStreamSynchronize(s->st);
-    (void)hipFree(save);
+    if (save) {
+        (void)hipMemcpyAsync(phi, save, n * sizeof(double), hipMemcpyDeviceToDevice, s->st);
+        (void)hipStreamSynchronize(s->st);
+        (void)hipFree(save);
+    }
     if ((rc = check_launch(s))) return rc;
-    const int nx = field == 2 ? L.npx - 1 : L.npx - 2, ny = L.npy - 1;
-    const double nunk = (double)(nx - 1) * (double)(ny - 1);
-    /* one launch per sweep (a Jacobi sweep, or both red-black colours) */
+    const int nx = field == 2 ? G.npx - 1 : G.npx - 2, ny = G.npy - 1;
+    const double nunk = (double)(nx - 1) * (double)(ny - 1) * (double)G.count;
+    /* one launch per sweep; algorithmic words per unknown: aw ae as an ap su
+     * and phi read + write (FP64), or aw ae as an 1/ap r and d read + write
+     * (FP32) */
     if (mean_us) *mean_us = 1000.0 * (double)ms / (double)even;
-    if (bytes_per_launch) *bytes_per_launch = 8.0 * 8.0 * nunk;
+    if (bytes_per_launch) *bytes_per_launch = 8.0 * (mixed ? 4.0 : 8.0) * nunk;
     return MGF_OK;
 }

@@ -310,3 +310,11 @@ class Batch:
         out = np.empty(nx * ny, np.float64)
         self._check(lib().mgf_batch_get_field(self.h, b, ig, name.encode(), _dptr(out)))
         return out.reshape(ny, nx)
+
+    def time_kernel(self, kernel: int, ig: int, reps: int):
+        """Mean microseconds and algorithmic bytes of one smoother-sweep
+        launch over every member's level at problem 0's depth of level ig."""
+        us, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().mgf_time_kernel(self.h, kernel, ig, reps, ctypes.byref(us),
+                                          ctypes.byref(b)))
+        return us.value, b.value
