@@ -43,6 +43,9 @@ sys.path.insert(0, os.path.join(ROOT, "multigrid-fvm_amd"))
 METRIC = ("converged solves/s at FP64 residual 1e-6, batched-mixed vs plain GPU; "
           "% HBM peak")
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_rb32 launch over the 512^2 members, from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE passes scaled by a 1 GiB device copy (bench.py --roofline-only)
+TRAFFIC = None
 WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
             "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
             "each obstacle moved by a seeded quarter/half/full cell), Re in "
@@ -90,6 +93,56 @@ def time_reference_cycle(prob, workdir, target_s=4.0):
     return (run(n) / n if n != probe else per), kind, n
 
 
+def sweep_roofline(probs, pairs, reps, cycles=None):
+    """Mean duration of one k_rb32 launch (a FP32 red-black sweep of the p'
+    equation over every member of the largest grid size, level 0), timed with
+    HIP events on the batch's own stream after that batch has run (so the
+    coefficients are the assembled ones).  Graph-replayed V-cycles cannot be
+    bracketed per kernel, so the launch is repeated on its own."""
+    import mgfvm
+    nmax = max(p.nfx for p in probs)
+    big = [k for k in range(len(probs)) if probs[k].nfx == nmax]
+    big_pairs = [pairs[k] for k in big]
+    if cycles is not None:
+        work = os.path.dirname(os.path.dirname(pairs[0][0]))
+        big_pairs = [probs[k].replace(nvcycle=cycles, convergence=0.0).write(
+            os.path.join(work, f"roof{k}")) for k in big]
+    bb = mgfvm.Batch(big_pairs, precision=mgfvm.PREC_MIXED, device=0)
+    bb.reset()
+    bb.solve()          # leaves assembled coefficients behind
+    us, nbytes = bb.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, reps)
+    bb.close()
+    achieved = nbytes / (us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
+            "kernel": f"k_rb32: one FP32 red-black sweep of the p' equation over the "
+                      f"{len(big_pairs)} {nmax}^2 members (one launch)",
+            "mean_us": round(us, 2), "bytes_per_launch": nbytes}
+
+
+def roofline_only(args):
+    """Counter-profiling run: the dominant kernel's launches and nothing else
+    of weight (a 3-cycle batch of the 512^2 members, then --reps sweeps),
+    followed by one device-to-device copy of 1 GiB whose counters scale
+    FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    from mgfvm.inputs import staggered_batch
+    torch.cuda.set_device(0)
+    work = tempfile.mkdtemp(prefix="mgf_roof_")
+    try:
+        probs = staggered_batch(args.count)
+        pairs = [p.write(os.path.join(work, f"p{k}")) for k, p in enumerate(probs)]
+        roof = sweep_roofline(probs, pairs, args.reps, cycles=3)
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        roof["copy_bytes"] = a.numel() * a.element_size()
+        print(json.dumps(roof))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +154,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-single", action="store_true", help="skip the 1024^2 single solve")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-at-a-time figure")
+    ap.add_argument("--reps", type=int, default=100, help="sweep launches timed for the roofline")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the dominant kernel and a 1 GiB copy (for counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -115,6 +171,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.roofline_only:
+        roofline_only(args)
+        return
     torch.cuda.set_device(0)
     work = tempfile.mkdtemp(prefix="mgf_bench_")
     try:
@@ -141,19 +200,7 @@ def main():
               flush=True)
 
         # ---- roofline: one FP32 sweep launch over the largest members ----
-        nmax = max(p.nfx for p in probs)
-        big_pairs = [pairs[k] for k in range(len(probs)) if probs[k].nfx == nmax]
-        bb = mgfvm.Batch(big_pairs, precision=mgfvm.PREC_MIXED, device=0)
-        bb.reset()
-        bb.solve()          # leaves assembled coefficients behind
-        us, nbytes = bb.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, 100)
-        bb.close()
-        achieved = nbytes / (us * 1e-6) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_rb32: one FP32 red-black sweep of the p' equation over the "
-                          f"{len(big_pairs)} {nmax}^2 members (one launch)",
-                "mean_us": round(us, 2), "bytes_per_launch": nbytes}
+        roof = sweep_roofline(probs, pairs, args.reps)
 
         # ---- plain FP64 GPU path: one problem at a time -----------------
         plain_value, ks = None, []

@@ -89,6 +89,15 @@ typedef struct mgf_options {
     int device;                /* HIP device ordinal (-1: current) */
     int strict_limits;         /* reject inputs beyond the reference's static limits
                                   (nobs <= 1000, ninlt/noutlt <= 10, nvcycle <= 3000) */
+    int sweeps_per_launch;     /* MGF_PREC_MIXED red-black: sweeps fused into one
+                                  temporally blocked launch (1-6; 0: automatic).  The
+                                  result does not depend on it. */
+    int sweep_tile_rows;       /* rows of those launches' tiles (16 or 32; 0: automatic);
+                                  result-neutral as well */
+    int split_assembly;        /* MGF_PREC_MIXED red-black: 1 = assemble each equation in
+                                  a kernel of its own that stores it in FP32; 0 (default)
+                                  = the first sweep launch assembles it on the fly.
+                                  Result-neutral. */
 } mgf_options;
 
 typedef struct mgf_solver mgf_solver;
@@ -190,14 +199,17 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
 /* Kernel ids for mgf_time_kernel */
 #define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation */
 #define MGF_KERNEL_SWEEP_U   1   /* one full smoother sweep of the u equation */
-/* Time `reps` launches of one smoother sweep of level ig (for a batch: the
- * launch covering every problem's level at the depth of problem 0's level
- * ig) with HIP events on the solver's stream, on the current coefficients.
- * FP64 mode works in place on a single problem and restores the field; mixed
- * mode sweeps only its correction buffers.  Returns the mean duration of one
- * launch in microseconds and the algorithmic bytes it moves (8 words per
- * unknown: 4 neighbour coefficients, diagonal, right-hand side, the unknown
- * read and written; 8 bytes each in FP64, 4 in FP32). */
+/* Time `reps` smoother launches of level ig (for a batch: the launch
+ * covering every problem's level at the depth of problem 0's level ig) with
+ * HIP events on the solver's stream, on the current coefficients.  A launch
+ * is one sweep, except for mixed-precision red-black, where it is the first
+ * launch of that field's relaxation: min(sweeps_per_launch, nswpm or nswpp)
+ * sweeps.  FP64 mode works in place on a single problem and restores the
+ * field; mixed mode sweeps only its correction buffers.  Returns the mean
+ * duration of one launch in microseconds and the algorithmic bytes of its
+ * sweeps (8 words per unknown and sweep: 4 neighbour coefficients,
+ * diagonal, right-hand side, the unknown read and written; 8 bytes each in
+ * FP64, 4 in FP32). */
 int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps,
                     double *mean_us, double *bytes_per_launch);
 

@@ -108,9 +108,24 @@ void mgf_launch_prolong(const DevPair &R, hipStream_t st);
 void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
 void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st);
-void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
-/* the FP32 sweeps alone (they only touch the correction buffers) */
-void mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
+/* shape of the temporally blocked red-black sweeps: sweeps per launch and
+ * tile rows */
+struct SweepCfg {
+    int spl, ty, fuse;
+};
+#define MGF_AUTO_SWEEPS_PER_LAUNCH 3
+#define MGF_MAX_SWEEPS_PER_LAUNCH 6
+#define MGF_AUTO_SWEEP_TILE_ROWS 16
+SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int tile_rows, int split_assembly);
+/* assembly + sweeps + FP64 update of one field in a relaxation */
+void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int nsw,
+                                  SweepCfg cfg, int record, hipStream_t st);
+void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
+                             hipStream_t st);
+/* the FP32 sweeps alone (they only touch the correction buffers); returns
+ * the number of launches */
+int mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
+                            hipStream_t st);
 void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st);
 int mgf_partial_blocks(int npx, int npy);
 

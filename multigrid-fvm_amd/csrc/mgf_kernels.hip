@@ -96,8 +96,8 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
     LEVEL(Ls);
     const DevPhys &P = L.P;
     __shared__ double lds[BY];
-    const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int i = 1 + (int)(T.x * BX + threadIdx.x);
+    const int j = 1 + (int)(T.y * BY + threadIdx.y);
     const int nx = L.npx;
     double mag = 0.0;
     if (i <= nx - 2 && j <= L.npy - 2) {
@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
         L.ap[ij] = ap;
     }
     double bs = block_sum(mag, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) L.partial[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+    const int pb = T.y * (int)gridDim.x + T.x;     /* padding rows have none */
+    if (threadIdx.x == 0 && threadIdx.y == 0 && pb < L.nblk) L.partial[pb] = bs;
 }
 
 /* sum of the per-block partials in a fixed order (one workgroup), so the
@@ -218,7 +219,7 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
     double *dst = parity ? (double *)base : L.tmp;
     __shared__ double t[RB_TY + 4][BX + 4];
     const int npx = L.npx, npy = L.npy;
-    const int i0 = 1 + (int)blockIdx.x * BX, j0 = 1 + (int)blockIdx.y * RB_TY;
+    const int i0 = 1 + T.x * BX, j0 = 1 + T.y * RB_TY;
     const int tid = threadIdx.y * BX + threadIdx.x;
     constexpr int NST = ((RB_TY + 4) * (BX + 4) + BX * BY - 1) / (BX * BY);
 
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
  * anything (the reference copies after its update loop, in list order) */
 __global__ void k_pstash(const DevLevel *Ls)
 {
-    LEVEL(Ls);
+    LEVEL1D(Ls);
     const DevPhys &P = L.P;
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (s >= L.npcopy) return;
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
 /* ---------------------------------------------------------------------- */
 __global__ void k_pbound_x(const DevLevel *Ls)
 {
-    LEVEL(Ls);
+    LEVEL1D(Ls);
     if (L.P.ibc[0] != 3) return;
     const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j > L.npy - 2) return;
@@ -372,7 +373,7 @@ __global__ void k_pbound_x(const DevLevel *Ls)
  * depend on each other; one thread runs them in the reference's order */
 __global__ void k_pbound_y(const DevLevel *Ls)
 {
-    LEVEL(Ls);
+    LEVEL1D(Ls);
     if (L.P.ibc[2] != 3) return;
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int nx = L.npx, ny = L.npy;
@@ -387,7 +388,7 @@ __global__ void k_pbound_y(const DevLevel *Ls)
 
 __global__ void k_outflow(const DevLevel *Ls)
 {
-    LEVEL(Ls);
+    LEVEL1D(Ls);
     if (L.P.ibc[1] != 4) return;
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (j > L.npy - 1) return;
@@ -406,8 +407,8 @@ __global__ void k_outflow(const DevLevel *Ls)
 __global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
-    const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-    const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int ic = 1 + (int)(T.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(T.y * BY + threadIdx.y);
     if (ic > C.npx - 2 || jc > C.npy - 2) return;
     const int nxf = F.npx;
     const int ijc = jc * C.npx + ic;
@@ -464,38 +465,71 @@ __device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fi
 }
 
 /* FAS tau terms resu/resv/ressc on the coarse level (V6:1547-1563,
- * 1632-1647, 1723-1739); the fine and coarse residuals are evaluated where
- * they are needed instead of through the rs scratch array */
+ * 1632-1647, 1723-1739).  A workgroup covers 64 x 4 coarse points: it first
+ * evaluates every fine residual they need once, at unit stride, into LDS
+ * (the reference's rs scratch array, V6:1496-1530, restricted to the
+ * workgroup's footprint), then each coarse point combines its fine
+ * residuals in the reference's order and subtracts its coarse residual. */
+#define TAU_UW (2 * BX + 1)
+#define TAU_W (2 * BX)
+#define TAU_H (2 * BY)
 __global__ void __launch_bounds__(BX * BY) k_tau(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
-    const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-    const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    __shared__ double U[TAU_H][TAU_UW];
+    __shared__ double V[TAU_H + 1][TAU_W];
+    __shared__ double Q[TAU_H][TAU_W];
     const int nxc = C.npx, nyc = C.npy;
-    if (ic > nxc - 2 || jc > nyc - 2) return;
-    const int ijc = jc * nxc + ic;
-    const int i = 2 * ic, j = 2 * jc;
+    const int ic0 = 1 + T.x * BX, jc0 = 1 + T.y * BY;
+    if (ic0 > nxc - 2 || jc0 > nyc - 2) return;          /* padding tiles */
+    const int fi0 = 2 * ic0 - 1, fj0 = 2 * jc0 - 1;
+    const int tid = threadIdx.y * BX + threadIdx.x;
     /* onto a level below the obstacle level the penalised fine residuals
      * are dropped (their nonzero value is balanced by the 1e30 penalty,
      * which the residual leaves out); never the case when ig_obs == ngrid */
     const bool m = C.ext != 0;
+    /* last coarse point of the workgroup for each equation, and the fine
+     * columns / rows its residual stencils reach (relative to fi0, fj0) */
+    const int icu = min(ic0 + BX - 1, nxc - 3), icc = min(ic0 + BX - 1, nxc - 2);
+    const int jcc = min(jc0 + BY - 1, nyc - 2), jcv = min(jc0 + BY - 1, nyc - 3);
+    const int cu = icu >= ic0 ? 2 * (icu - ic0) + 2 : -1, ru = 2 * (jcc - jc0) + 1;
+    const int cv = 2 * (icc - ic0) + 1, rv = jcv >= jc0 ? 2 * (jcv - jc0) + 2 : -1;
+    const int cq = 2 * (icc - ic0) + 1, rq = 2 * (jcc - jc0) + 1;
+    for (int e = tid; e < TAU_H * TAU_UW; e += BX * BY) {
+        const int r = e / TAU_UW, c = e - r * TAU_UW;
+        U[r][c] = (c <= cu && r <= ru) ? rs_u(F, fi0 + c, fj0 + r, F.bodyx, true, m) : 0.0;
+    }
+    for (int e = tid; e < (TAU_H + 1) * TAU_W; e += BX * BY) {
+        const int r = e / TAU_W, c = e - r * TAU_W;
+        V[r][c] = (c <= cv && r <= rv) ? rs_v(F, fi0 + c, fj0 + r, F.bodyy, true, m) : 0.0;
+    }
+    for (int e = tid; e < TAU_H * TAU_W; e += BX * BY) {
+        const int r = e / TAU_W, c = e - r * TAU_W;
+        Q[r][c] = (c <= cq && r <= rq) ? rs_sc(F, fi0 + c, fj0 + r, true, m) : 0.0;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int ic = ic0 + tx, jc = jc0 + ty;
+    if (ic > nxc - 2 || jc > nyc - 2) return;
+    const int ijc = jc * nxc + ic;
+    const int c1 = 2 * tx + 1, r1 = 2 * ty + 1;     /* fine (2ic, 2jc) in the tables */
     if (ic <= nxc - 3) {
-        double a = rs_u(F, i, j, F.bodyx, true, m), b = rs_u(F, i, j - 1, F.bodyx, true, m);
-        double w0 = rs_u(F, i - 1, j, F.bodyx, true, m), w1 = rs_u(F, i - 1, j - 1, F.bodyx, true, m);
-        double e0 = rs_u(F, i + 1, j, F.bodyx, true, m), e1 = rs_u(F, i + 1, j - 1, F.bodyx, true, m);
+        double a = U[r1][c1], b = U[r1 - 1][c1];
+        double w0 = U[r1][c1 - 1], w1 = U[r1 - 1][c1 - 1];
+        double e0 = U[r1][c1 + 1], e1 = U[r1 - 1][c1 + 1];
         double rc = rs_u(C, ic, jc, C.bodyx, false);
         C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rc;
     }
     if (jc <= nyc - 3) {
-        double a = rs_v(F, i, j, F.bodyy, true, m), b = rs_v(F, i - 1, j, F.bodyy, true, m);
-        double s0 = rs_v(F, i, j - 1, F.bodyy, true, m), s1 = rs_v(F, i - 1, j - 1, F.bodyy, true, m);
-        double n0 = rs_v(F, i, j + 1, F.bodyy, true, m), n1 = rs_v(F, i - 1, j + 1, F.bodyy, true, m);
+        double a = V[r1][c1], b = V[r1][c1 - 1];
+        double s0 = V[r1 - 1][c1], s1 = V[r1 - 1][c1 - 1];
+        double n0 = V[r1 + 1][c1], n1 = V[r1 + 1][c1 - 1];
         double rc = rs_v(C, ic, jc, C.bodyy, false);
         C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rc;
     }
     {
-        double a = rs_sc(F, i, j, true, m), b = rs_sc(F, i, j - 1, true, m);
-        double c2 = rs_sc(F, i - 1, j, true, m), d = rs_sc(F, i - 1, j - 1, true, m);
+        double a = Q[r1][c1], b = Q[r1 - 1][c1];
+        double c2 = Q[r1][c1 - 1], d = Q[r1 - 1][c1 - 1];
         double rc = rs_sc(C, ic, jc, false);
         C.ressc[ijc] = (((a + b) + c2) + d) - rc;
     }
@@ -512,8 +546,8 @@ __global__ void __launch_bounds__(BX * BY) k_tau(const DevLevel *Fs, const DevLe
 __global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
-    const int ic = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-    const int jc = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int ic = 1 + (int)(T.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(T.y * BY + threadIdx.y);
     const int nxc = C.npx, nyc = C.npy, nxf = F.npx;
     if (ic > nxc - 2 || jc > nyc - 2) return;
     const int ijc = jc * nxc + ic;
@@ -571,59 +605,89 @@ __device__ __forceinline__ double corr_v(const DevLevel &F, const DevLevel &C, i
     return c;
 }
 
-/* bilinear 9/16, 3/16, 3/16, 1/16 cell correction (V6:1945-1955) */
-__device__ __forceinline__ double corr_cell(const double *c, int nxc, int ic, int jc, int di, int dj)
+/* bilinear 9/16, 3/16, 3/16, 1/16 correction (V6:1945-1955) of the fine
+ * cell on side (di, dj) of a coarse cell, from the coarse correction's 3 x 3
+ * neighbourhood n[dj+1][di+1] */
+__device__ __forceinline__ double corr_cell_n(const double n[3][3], int di, int dj)
 {
     const double f1 = 9. / 16., f2 = 3. / 16., f3 = 3. / 16., f4 = 1. / 16.;
-    const int ijc = jc * nxc + ic;
-    return ((f1 * c[ijc] + f2 * c[ijc + di]) + f3 * c[ijc + dj * nxc]) + f4 * c[ijc + di + dj * nxc];
+    return ((f1 * n[1][1] + f2 * n[1][1 + di]) + f3 * n[1 + dj][1]) + f4 * n[1 + dj][1 + di];
 }
 
-/* every fine point gathers its corrections; odd faces, which the reference
- * updates twice (atomics in its GPU build), take the two halves in its CPU
- * loop order (smaller coarse index first) */
+/* Every fine point gathers its corrections; a thread takes the four fine
+ * cells (and their faces) of one coarse cell, so the coarse values are
+ * loaded once for four points.  Odd faces, which the reference updates
+ * twice (atomics in its GPU build), take the two halves in its CPU loop
+ * order (smaller coarse index first). */
 __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
-    const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
-    const int nxf = F.npx, nyf = F.npy, nxc = C.npx, nyc = C.npy;
-    if (i > nxf - 2 || j > nyf - 2) return;
-    const int ij = j * nxf + i;
-    /* u faces: i in [1, nxf-3], coarse faces ic in [1, nxc-3] */
-    if (i <= nxf - 3) {
-        const int jc = (j + 1) >> 1;
-        const bool upper = !(j & 1);              /* row 2jc: corr1; row 2jc-1: corr2 */
-        double u = F.u[ij];
-        if (!(i & 1)) {
-            u = u + corr_u(F, C, i >> 1, jc, upper);
-        } else {
-            const int icl = (i - 1) >> 1, icr = (i + 1) >> 1;
-            if (icl >= 1) u = u + 0.5 * corr_u(F, C, icl, jc, upper);
-            if (icr <= nxc - 3) u = u + 0.5 * corr_u(F, C, icr, jc, upper);
+    const int ic = 1 + (int)(T.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(T.y * BY + threadIdx.y);
+    const int nxf = F.npx, nxc = C.npx, nyc = C.npy;
+    if (ic > nxc - 2 || jc > nyc - 2) return;
+    const int i1 = 2 * ic - 1, i2 = 2 * ic;
+    const int r1 = (2 * jc - 1) * nxf, r2 = (2 * jc) * nxf;
+    /* u faces i1 (odd: halves from coarse faces ic-1 and ic) and i2 = 2ic
+     * (coarse face ic, if it is an interior face); rows 2jc (upper) and
+     * 2jc-1 */
+    {
+        const bool hr = ic <= nxc - 3, hl = ic >= 2;
+        double crU = 0.0, crL = 0.0, clU = 0.0, clL = 0.0;
+        if (hr) { crU = corr_u(F, C, ic, jc, true); crL = corr_u(F, C, ic, jc, false); }
+        if (hl) { clU = corr_u(F, C, ic - 1, jc, true); clL = corr_u(F, C, ic - 1, jc, false); }
+        double u = F.u[r2 + i1];
+        if (hl) u = u + 0.5 * clU;
+        if (hr) u = u + 0.5 * crU;
+        F.u[r2 + i1] = u;
+        u = F.u[r1 + i1];
+        if (hl) u = u + 0.5 * clL;
+        if (hr) u = u + 0.5 * crL;
+        F.u[r1 + i1] = u;
+        if (hr) {
+            F.u[r2 + i2] = F.u[r2 + i2] + crU;
+            F.u[r1 + i2] = F.u[r1 + i2] + crL;
         }
-        F.u[ij] = u;
     }
-    /* v faces: j in [1, nyf-3], coarse faces jc in [1, nyc-3] */
-    if (j <= nyf - 3) {
-        const int ic = (i + 1) >> 1;
-        const bool right = !(i & 1);              /* column 2ic: corr1; 2ic-1: corr2 */
-        double v = F.v[ij];
-        if (!(j & 1)) {
-            v = v + corr_v(F, C, ic, j >> 1, right);
-        } else {
-            const int jcl = (j - 1) >> 1, jcu = (j + 1) >> 1;
-            if (jcl >= 1) v = v + 0.5 * corr_v(F, C, ic, jcl, right);
-            if (jcu <= nyc - 3) v = v + 0.5 * corr_v(F, C, ic, jcu, right);
+    /* v faces: row 2jc-1 (odd: halves from coarse faces jc-1 and jc) and row
+     * 2jc (coarse face jc, if interior); columns 2ic (right) and 2ic-1 */
+    {
+        const bool hu = jc <= nyc - 3, hd = jc >= 2;
+        double cuR = 0.0, cuL = 0.0, cdR = 0.0, cdL = 0.0;
+        if (hu) { cuR = corr_v(F, C, ic, jc, true); cuL = corr_v(F, C, ic, jc, false); }
+        if (hd) { cdR = corr_v(F, C, ic, jc - 1, true); cdL = corr_v(F, C, ic, jc - 1, false); }
+        double v = F.v[r1 + i2];
+        if (hd) v = v + 0.5 * cdR;
+        if (hu) v = v + 0.5 * cuR;
+        F.v[r1 + i2] = v;
+        v = F.v[r1 + i1];
+        if (hd) v = v + 0.5 * cdL;
+        if (hu) v = v + 0.5 * cuL;
+        F.v[r1 + i1] = v;
+        if (hu) {
+            F.v[r2 + i2] = F.v[r2 + i2] + cuR;
+            F.v[r2 + i1] = F.v[r2 + i1] + cuL;
         }
-        F.v[ij] = v;
     }
     /* cells */
     {
-        const int ic = (i + 1) >> 1, jc = (j + 1) >> 1;
-        const int di = (i & 1) ? -1 : 1, dj = (j & 1) ? -1 : 1;
-        F.p[ij] = F.p[ij] + corr_cell(C.cp, nxc, ic, jc, di, dj);
-        F.sc[ij] = F.sc[ij] + corr_cell(C.csc, nxc, ic, jc, di, dj);
+        const int ijc = jc * nxc + ic;
+        double np[3][3], ns[3][3];
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+            for (int di = -1; di <= 1; ++di) {
+                np[dj + 1][di + 1] = C.cp[ijc + dj * nxc + di];
+                ns[dj + 1][di + 1] = C.csc[ijc + dj * nxc + di];
+            }
+        F.p[r1 + i1] = F.p[r1 + i1] + corr_cell_n(np, -1, -1);
+        F.p[r1 + i2] = F.p[r1 + i2] + corr_cell_n(np, 1, -1);
+        F.p[r2 + i1] = F.p[r2 + i1] + corr_cell_n(np, -1, 1);
+        F.p[r2 + i2] = F.p[r2 + i2] + corr_cell_n(np, 1, 1);
+        F.sc[r1 + i1] = F.sc[r1 + i1] + corr_cell_n(ns, -1, -1);
+        F.sc[r1 + i2] = F.sc[r1 + i2] + corr_cell_n(ns, 1, -1);
+        F.sc[r2 + i1] = F.sc[r2 + i1] + corr_cell_n(ns, -1, 1);
+        F.sc[r2 + i2] = F.sc[r2 + i2] + corr_cell_n(ns, 1, 1);
     }
 }
 
@@ -659,26 +723,22 @@ int mgf_partial_blocks(int npx, int npy)
     return (int)(g.x * g.y);
 }
 
-static inline dim3 grid3d(int ni, int nj, int nz, int by = BY)
-{
-    return dim3((unsigned)((ni + BX - 1) / BX), (unsigned)((nj + by - 1) / by), (unsigned)nz);
-}
 
 void mgf_launch_fluxes(const DevGroup &G, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fluxes, grid3d(G.npx - 1, G.npy - 1, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_fluxes, tile_grid(G.npx - 1, G.npy - 1, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
 void mgf_launch_asm_u(const DevGroup &G, hipStream_t st)
 {
     if (G.npx - 3 < 1) return;
-    hipLaunchKernelGGL(k_asm_u, grid3d(G.npx - 3, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_asm_u, tile_grid(G.npx - 3, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
 void mgf_launch_asm_v(const DevGroup &G, hipStream_t st)
 {
     if (G.npy - 3 < 1) return;
-    hipLaunchKernelGGL(k_asm_v, grid3d(G.npx - 2, G.npy - 3, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_asm_v, tile_grid(G.npx - 2, G.npy - 3, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
 void mgf_launch_reduce_err(const DevGroup &G, int record, hipStream_t st)
@@ -689,13 +749,13 @@ void mgf_launch_reduce_err(const DevGroup &G, int record, hipStream_t st)
 
 void mgf_launch_asm_p(const DevGroup &G, int record, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_asm_p, grid3d(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_asm_p, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
     mgf_launch_reduce_err(G, record, st);
 }
 
 void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_asm_sc, grid3d(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_asm_sc, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
@@ -709,14 +769,14 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
     if (nx - 1 < 1 || ny - 1 < 1 || nsw <= 0 || (isolve != 1 && isolve != 2)) return;
     for (int it = 0; it < nsw; ++it) {
         if (isolve == 1)
-            hipLaunchKernelGGL(k_jacobi, grid3d(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
+            hipLaunchKernelGGL(k_jacobi, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
                                field, it & 1);
         else
-            hipLaunchKernelGGL(k_rb_sweep, grid3d(nx - 1, ny - 1, G.count, RB_TY), dim3(BX, BY),
+            hipLaunchKernelGGL(k_rb_sweep, tile_grid(nx - 1, ny - 1, G.count, RB_TY), dim3(BX, BY),
                                0, st, G.d, field, it & 1);
     }
     if (nsw & 1)
-        hipLaunchKernelGGL(k_copy_range, grid3d(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+        hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, field);
 }
 
@@ -726,7 +786,7 @@ void mgf_launch_update(const DevGroup &G, hipStream_t st)
         hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
-    hipLaunchKernelGGL(k_update, grid3d(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    hipLaunchKernelGGL(k_update, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st)
@@ -747,16 +807,16 @@ void mgf_launch_outflow(const DevGroup &G, hipStream_t st)
 
 void mgf_launch_restrict(const DevPair &R, hipStream_t st)
 {
-    dim3 g = grid3d(R.npxc - 2, R.npyc - 2, R.count);
+    dim3 g = tile_grid(R.npxc - 2, R.npyc - 2, R.count);
     hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, R.f, R.c);
     hipLaunchKernelGGL(k_tau, g, dim3(BX, BY), 0, st, R.f, R.c);
 }
 
 void mgf_launch_prolong(const DevPair &R, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_prol_c, grid3d(R.npxc - 2, R.npyc - 2, R.count), dim3(BX, BY), 0, st,
+    hipLaunchKernelGGL(k_prol_c, tile_grid(R.npxc - 2, R.npyc - 2, R.count), dim3(BX, BY), 0, st,
                        R.f, R.c);
-    hipLaunchKernelGGL(k_prol_apply, grid3d(R.npxf - 2, R.npyf - 2, R.count), dim3(BX, BY), 0,
+    hipLaunchKernelGGL(k_prol_apply, tile_grid(R.npxc - 2, R.npyc - 2, R.count), dim3(BX, BY), 0,
                        st, R.f, R.c);
 }
 

@@ -21,17 +21,33 @@
 
 #include "mgf_stencil.h"
 
-/* FP64 assembly of one equation at one point plus its FP64 residual, stored
- * in FP32 (field 0 u, 1 v, 3 sc; p' has its own kernel below) */
-__global__ void __launch_bounds__(BX * BY) k_asm_mx(const DevLevel *Ls, int field)
+struct Pt32 {
+    float aw, ae, as, an, iap, r;
+};
+
+__device__ __forceinline__ Pt32 load_pt(const DevLevel &L, int g)
 {
-    LEVEL(Ls);
+    return Pt32{L.aw32[g], L.ae32[g], L.as32[g], L.an32[g], L.iap32[g], L.r32[g]};
+}
+
+__device__ __forceinline__ void store_pt(const DevLevel &L, int g, const Pt32 &q)
+{
+    L.aw32[g] = q.aw;
+    L.ae32[g] = q.ae;
+    L.as32[g] = q.as;
+    L.an32[g] = q.an;
+    L.iap32[g] = q.iap;
+    L.r32[g] = q.r;
+}
+
+/* FP64 assembly of one equation at one point plus its FP64 residual, rounded
+ * to FP32 (field 0 u, 1 v, 3 sc); ap_out = the FP64 diagonal, which the
+ * caller stores as apu / apv for u / v.  No global stores here, so a caller
+ * assembling many points can keep all their loads in flight. */
+__device__ __forceinline__ Pt32 asm_pt(const DevLevel &L, int field, int i, int j, double &ap_out)
+{
     const DevPhys &P = L.P;
     const int nx = L.npx;
-    int i0 = 1, i1 = nx - 2, j1 = L.npy - 2;
-    if (field == 0) i1 = nx - 3;
-    if (field == 1) j1 = L.npy - 3;
-    RANGE2D(i0, i1, 1, j1);
     const int ij = j * nx + i;
     Coef c;
     double su, ap;
@@ -42,7 +58,6 @@ __global__ void __launch_bounds__(BX * BY) k_asm_mx(const DevLevel *Ls, int fiel
         ap = c.ap / P.relxm;
         su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
         ap = ap + ((L.flags[ij] & MGF_F_SPU) ? 1.0e30 : 0.0);
-        L.apu[ij] = ap;
         phi = L.u;
     } else if (field == 1) {
         c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
@@ -50,7 +65,6 @@ __global__ void __launch_bounds__(BX * BY) k_asm_mx(const DevLevel *Ls, int fiel
         ap = c.ap / P.relxm;
         su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
         ap = ap + ((L.flags[ij] & MGF_F_SPV) ? 1.0e30 : 0.0);
-        L.apv[ij] = ap;
         phi = L.v;
     } else {
         const uint8_t fl = L.flags[ij];
@@ -62,49 +76,66 @@ __global__ void __launch_bounds__(BX * BY) k_asm_mx(const DevLevel *Ls, int fiel
     }
     const double r = ((((su + c.aw * phi[ij - 1]) + c.ae * phi[ij + 1]) + c.as * phi[ij - nx]) +
                       c.an * phi[ij + nx]) - ap * phi[ij];
-    L.aw32[ij] = (float)c.aw;
-    L.ae32[ij] = (float)c.ae;
-    L.as32[ij] = (float)c.as;
-    L.an32[ij] = (float)c.an;
-    L.iap32[ij] = (float)(L.isolve == 1 ? 1.0 / (ap + 1.0e-30) : 1.0 / ap);
-    L.r32[ij] = (float)r;
+    ap_out = ap;
+    return Pt32{(float)c.aw, (float)c.ae, (float)c.as, (float)c.an,
+                (float)(L.isolve == 1 ? 1.0 / (ap + 1.0e-30) : 1.0 / ap), (float)r};
 }
 
-/* masserror + coefp in FP64 (V6:1018-1125), stored in FP32; partial sums of
- * |div| in FP64 for the deterministic reduction */
+/* masserror + coefp in FP64 (V6:1018-1125) at one cell, rounded to FP32;
+ * mag = |div| */
+__device__ __forceinline__ Pt32 asm_pt_p(const DevLevel &L, int i, int j, double &mag)
+{
+    const DevPhys &P = L.P;
+    const int nx = L.npx;
+    const int ij = j * nx + i, ijs = ij - nx;
+    double div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
+    double su = -div;
+    mag = fabs(div);
+    double aw = L.axax / L.apu[ij - 1];
+    double ae = L.axax / L.apu[ij];
+    double as = L.ayay / L.apv[ijs];
+    double an = L.ayay / L.apv[ij];
+    double ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
+    const double diag = 1.0e30;
+    if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
+    if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
+    if (P.ibc[2] == 2 && j == 1) { ap = ap + diag; su = 0.0; }
+    if (P.ibc[3] == 2 && j == L.npy - 2) { ap = ap + diag; su = 0.0; }
+    return Pt32{(float)aw, (float)ae, (float)as, (float)an,
+                (float)(L.isolve == 1 ? 1.0 / (ap + 1.0e-30) : 1.0 / ap), (float)su};
+}
+
+/* assembly of u, v or sc over the swept range, stored for the sweeps */
+__global__ void __launch_bounds__(BX * BY) k_asm_mx(const DevLevel *Ls, int field)
+{
+    LEVEL(Ls);
+    const int nx = L.npx;
+    int i0 = 1, i1 = nx - 2, j1 = L.npy - 2;
+    if (field == 0) i1 = nx - 3;
+    if (field == 1) j1 = L.npy - 3;
+    RANGE2D(i0, i1, 1, j1);
+    const int ij = j * nx + i;
+    double ap;
+    const Pt32 q = asm_pt(L, field, i, j, ap);
+    if (field == 0) L.apu[ij] = ap;
+    if (field == 1) L.apv[ij] = ap;
+    store_pt(L, ij, q);
+}
+
+/* assembly of p' over the cells, stored for the sweeps; partial sums of
+ * |div| in FP64 per 64 x 4 block for the deterministic reduction */
 __global__ void __launch_bounds__(BX * BY) k_asm_p_mx(const DevLevel *Ls)
 {
     LEVEL(Ls);
-    const DevPhys &P = L.P;
     __shared__ double lds[BY];
-    const int i = 1 + (int)(blockIdx.x * BX + threadIdx.x);
-    const int j = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int i = 1 + (int)(T.x * BX + threadIdx.x);
+    const int j = 1 + (int)(T.y * BY + threadIdx.y);
     const int nx = L.npx;
     double mag = 0.0;
-    if (i <= nx - 2 && j <= L.npy - 2) {
-        const int ij = j * nx + i, ijs = ij - nx;
-        double div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
-        double su = -div;
-        mag = fabs(div);
-        double aw = L.axax / L.apu[ij - 1];
-        double ae = L.axax / L.apu[ij];
-        double as = L.ayay / L.apv[ijs];
-        double an = L.ayay / L.apv[ij];
-        double ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
-        const double diag = 1.0e30;
-        if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
-        if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
-        if (P.ibc[2] == 2 && j == 1) { ap = ap + diag; su = 0.0; }
-        if (P.ibc[3] == 2 && j == L.npy - 2) { ap = ap + diag; su = 0.0; }
-        L.aw32[ij] = (float)aw;
-        L.ae32[ij] = (float)ae;
-        L.as32[ij] = (float)as;
-        L.an32[ij] = (float)an;
-        L.iap32[ij] = (float)(L.isolve == 1 ? 1.0 / (ap + 1.0e-30) : 1.0 / ap);
-        L.r32[ij] = (float)su;
-    }
+    if (i <= nx - 2 && j <= L.npy - 2) store_pt(L, j * nx + i, asm_pt_p(L, i, j, mag));
     double bs = block_sum(mag, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) L.partial[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+    const int pb = T.y * (int)gridDim.x + T.x;     /* padding rows have none */
+    if (threadIdx.x == 0 && threadIdx.y == 0 && pb < L.nblk) L.partial[pb] = bs;
 }
 
 /* extents of the swept range of a field (the reference's solver arguments) */
@@ -113,117 +144,143 @@ __device__ __forceinline__ void extents(const DevLevel &L, int field, int &nx, i
     (void)field_of(L, field, nx, ny);
 }
 
-/* One FP32 red-black sweep (both colours) of A d = r on a 64 x RB_TY tile,
- * d read from buffer it&1 (zero on the first sweep and outside the swept
- * range) and written to buffer (it+1)&1.  Same tiling as the FP64 sweep. */
-#define RB_TY 16
-#define RB_Q (RB_TY / BY)
-#define RB_RING (2 * (BX + 2) + 2 * RB_TY)
-
-struct Pt32 {
-    float aw, ae, as, an, iap, r;
-};
-
-__device__ __forceinline__ void ring_pos(int k, int &r, int &c)
-{
-    if (k < BX + 2) { r = 1; c = 1 + k; }
-    else if (k < 2 * (BX + 2)) { r = RB_TY + 2; c = 1 + (k - (BX + 2)); }
-    else if (k < 2 * (BX + 2) + RB_TY) { r = 2 + (k - 2 * (BX + 2)); c = 1; }
-    else { r = 2 + (k - 2 * (BX + 2) - RB_TY); c = BX + 2; }
-}
-
-__device__ __forceinline__ Pt32 load_pt(const DevLevel &L, int g)
-{
-    return Pt32{L.aw32[g], L.ae32[g], L.as32[g], L.an32[g], L.iap32[g], L.r32[g]};
-}
-
-__global__ void __launch_bounds__(BX * BY) k_rb32(const DevLevel *Ls, int field, int it)
+/* S FP32 red-black sweeps (2S colour steps) of A d = r in one launch,
+ * temporally blocked.  The workgroup takes its 64 x TY tile with a halo of
+ * H = 2S points on every side (equation in registers, d in LDS); colour step
+ * m updates the points at distance >= m from the edge of that region, so
+ * after 2S steps the tile itself holds exactly what S single sweeps give
+ * (same expressions, same order: the result is bit-identical).  d crosses
+ * HBM once per launch instead of once per sweep.  d is read from buffer
+ * launch&1 (zero for launch 0 and outside the swept range) and written to
+ * the other buffer, or, for p' on the last launch, widened into pp.
+ *
+ * MODE_LOAD: the equation comes from the FP32 arrays of k_asm_mx /
+ * k_asm_p_mx.  MODE_ASM (launch 0 only): the kernel assembles it itself in
+ * FP64 from the state, for the halo too (recomputed by each workgroup that
+ * needs it), so the FP32 equation never crosses HBM; the tile's apu / apv
+ * and, for p', the |div| partial sums (per 64 x 4 block, in block_sum's
+ * order, so they equal k_asm_p_mx's bit for bit) are written as by the
+ * assembly kernels.  MODE_ASM_STORE: MODE_ASM that also stores the tile's
+ * FP32 equation for the launches that follow.  Nothing the assembly reads is
+ * written by the launch, so halo recomputation is race-free. */
+#define MODE_LOAD 0
+#define MODE_ASM 1
+#define MODE_ASM_STORE 2
+template <int S, int TY, int MODE>
+__global__ void __launch_bounds__(BX * BY) k_rb32x(const DevLevel *Ls, int field, int launch,
+                                                   int last)
 {
     LEVEL(Ls);
-    __shared__ float t[RB_TY + 4][BX + 4];
+    constexpr int H = 2 * S, RW = BX + 2 * H, RH = TY + 2 * H, NR = RW * RH;
+    constexpr int NT = BX * BY, NPT = (NR + NT - 1) / NT;
+    constexpr bool ASM = MODE != MODE_LOAD;
+    constexpr int MR = ASM ? TY : 1;
+    __shared__ float t[NR];
+    __shared__ double mg[MR][BX];
+    __shared__ double rows[MR];
     int nx, ny;
     extents(L, field, nx, ny);
-    const float *src = it == 0 ? nullptr : ((it & 1) ? L.db32 : L.da32);
-    float *dst = (it & 1) ? L.da32 : L.db32;
+    const float *src = launch == 0 ? nullptr : ((launch & 1) ? L.db32 : L.da32);
+    float *dst = (launch & 1) ? L.da32 : L.db32;
+    const bool pfield = field == 2;
     const int npx = L.npx;
-    const int i0 = 1 + (int)blockIdx.x * BX, j0 = 1 + (int)blockIdx.y * RB_TY;
+    const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * TY - H;
     const int tid = threadIdx.y * BX + threadIdx.x;
-    constexpr int NST = ((RB_TY + 4) * (BX + 4) + BX * BY - 1) / (BX * BY);
-    float sv[NST];
+    if (ASM && pfield) {
+        for (int k = tid; k < MR * BX; k += NT) mg[k / BX][k % BX] = 0.0;
+        __syncthreads();
+    }
+    Pt32 pt[NPT];
+    /* the tile's FP64 diagonals of u / v (stored as apu / apv at the end:
+     * no global store between the assembly loads) */
+    double apk[ASM ? NPT : 1];
+    /* per slot: distance to the region's edge (bits 0-7), colour (bit 8),
+     * inside the swept range (bit 9) */
+    unsigned meta[NPT];
 #pragma unroll
-    for (int q = 0; q < NST; ++q) {
-        const int k = tid + q * BX * BY;
+    for (int k = 0; k < NPT; ++k) {
+        const int p = tid + k * NT;
         float val = 0.f;
-        if (src && k < (RB_TY + 4) * (BX + 4)) {
-            const int r = k / (BX + 4), c = k % (BX + 4);
-            const int gi = i0 - 2 + c, gj = j0 - 2 + r;
-            if (gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1) val = src[gj * npx + gi];
+        unsigned m = 0;
+        pt[k] = Pt32{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (ASM) apk[k] = 0.0;
+        if (p < NR) {
+            const int r = p / RW, c = p - r * RW;
+            const int gi = ri0 + c, gj = rj0 + r;
+            if (gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1) {
+                const int g = gj * npx + gi;
+                const int d = min(min(c, RW - 1 - c), min(r, RH - 1 - r));
+                if (ASM) {
+                    if (pfield) {
+                        double mag;
+                        pt[k] = asm_pt_p(L, gi, gj, mag);
+                        if (d >= H) mg[r - H][c - H] = mag;
+                    } else {
+                        pt[k] = asm_pt(L, field, gi, gj, apk[ASM ? k : 0]);
+                    }
+                } else {
+                    pt[k] = load_pt(L, g);
+                }
+                if (src) val = src[g];
+                m = (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
+            }
+            t[p] = val;
         }
-        sv[q] = val;
-    }
-    const int gi = i0 + threadIdx.x;
-    Pt32 own[RB_Q];
-    bool live[RB_Q];
-#pragma unroll
-    for (int q = 0; q < RB_Q; ++q) {
-        const int gj = j0 + threadIdx.y + BY * q;
-        live[q] = (gi <= nx - 1 && gj <= ny - 1);
-        own[q] = live[q] ? load_pt(L, gj * npx + gi) : Pt32{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    }
-    int rr = 0, rc = 0;
-    bool ring = false;
-    Pt32 rp = Pt32{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (tid < RB_RING) {
-        ring_pos(tid, rr, rc);
-        const int ri = i0 - 2 + rc, rj = j0 - 2 + rr;
-        ring = (((ri + rj) & 1) == 0) && ri >= 1 && ri <= nx - 1 && rj >= 1 && rj <= ny - 1;
-        if (ring) rp = load_pt(L, rj * npx + ri);
-    }
-#pragma unroll
-    for (int q = 0; q < NST; ++q) {
-        const int k = tid + q * BX * BY;
-        if (k < (RB_TY + 4) * (BX + 4)) t[k / (BX + 4)][k % (BX + 4)] = sv[q];
+        meta[k] = m;
     }
     __syncthreads();
-    float res[RB_Q];
 #pragma unroll
-    for (int q = 0; q < RB_Q; ++q) {
-        const int r = 2 + threadIdx.y + BY * q, c = 2 + threadIdx.x;
-        const int gj = j0 + threadIdx.y + BY * q;
-        res[q] = t[r][c];
-        if (live[q] && ((gi + gj) & 1) == 0) {
-            const Pt32 &o = own[q];
-            const float s = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
-                            o.an * t[r + 1][c];
-            res[q] = (s + o.r) * o.iap;
+    for (int step = 1; step <= 2 * S; ++step) {
+        const unsigned want = (1u << 9) | ((unsigned)((step - 1) & 1) << 8);
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const unsigned m = meta[k];
+            if ((m & 0x300u) == want && (int)(m & 0xffu) >= step) {
+                const int p = tid + k * NT;
+                const Pt32 &o = pt[k];
+                const float s = ((o.aw * t[p - 1] + o.ae * t[p + 1]) + o.as * t[p - RW]) +
+                                o.an * t[p + RW];
+                t[p] = (s + o.r) * o.iap;
+            }
+        }
+        __syncthreads();
+    }
+    const bool to_pp = last && pfield;
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const unsigned m = meta[k];
+        if ((m & (1u << 9)) && (int)(m & 0xffu) >= H) {
+            const int p = tid + k * NT;
+            const int r = p / RW, c = p - r * RW;
+            const int g = (rj0 + r) * npx + (ri0 + c);
+            if (to_pp)
+                L.pp[g] = (double)t[p];
+            else
+                dst[g] = t[p];
+            if (ASM && field == 0) L.apu[g] = apk[ASM ? k : 0];
+            if (ASM && field == 1) L.apv[g] = apk[ASM ? k : 0];
+            if (MODE == MODE_ASM_STORE) store_pt(L, g, pt[k]);
         }
     }
-    float rv = 0.f;
-    if (ring) {
-        const float s = ((rp.aw * t[rr][rc - 1] + rp.ae * t[rr][rc + 1]) + rp.as * t[rr - 1][rc]) +
-                        rp.an * t[rr + 1][rc];
-        rv = (s + rp.r) * rp.iap;
-    }
+    if (ASM && pfield) {
+        /* per 64 x 4 block of the tile: wave w sums row w of the block with
+         * block_sum's shuffle tree, then the four rows in order */
+        const int w = threadIdx.y, lane = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < RB_Q; ++q) {
-        const int gj = j0 + threadIdx.y + BY * q;
-        if (live[q] && ((gi + gj) & 1) == 0) t[2 + threadIdx.y + BY * q][2 + threadIdx.x] = res[q];
-    }
-    if (ring) t[rr][rc] = rv;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < RB_Q; ++q) {
-        const int r = 2 + threadIdx.y + BY * q, c = 2 + threadIdx.x;
-        const int gj = j0 + threadIdx.y + BY * q;
-        if (!live[q]) continue;
-        float val = res[q];
-        if (((gi + gj) & 1) == 1) {
-            const Pt32 &o = own[q];
-            const float s = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
-                            o.an * t[r + 1][c];
-            val = (s + o.r) * o.iap;
+        for (int q = 0; q < TY / BY; ++q) {
+            double v = mg[q * BY + w][lane];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) rows[q * BY + w] = v;
         }
-        dst[gj * npx + gi] = val;
+        __syncthreads();
+        if (tid < TY / BY) {
+            const int q = tid;
+            const double bs = ((rows[q * BY] + rows[q * BY + 1]) + rows[q * BY + 2]) +
+                              rows[q * BY + 3];
+            const int gxp = (npx - 2 + BX - 1) / BX;
+            const int pb = (T.y * (TY / BY) + q) * gxp + T.x;
+            if (pb < L.nblk) L.partial[pb] = bs;
+        }
     }
 }
 
@@ -248,17 +305,18 @@ __global__ void __launch_bounds__(BX * BY) k_jac32(const DevLevel *Ls, int field
     dst[ij] = (s + L.r32[ij]) * L.iap32[ij];
 }
 
-/* phi += d (u, v, sc) or pp = d, in FP64, over the swept range */
-__global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int field, int nsw)
+/* phi += d (u, v, sc) or pp = d, in FP64, over the swept range; nl is the
+ * number of sweep launches (its parity names the buffer holding d) */
+__global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int field, int nl)
 {
     LEVEL(Ls);
     int nx, ny;
     double *phi = field_of(L, field, nx, ny);
     RANGE2D(1, nx - 1, 1, ny - 1);
-    const float *d = (nsw & 1) ? L.db32 : L.da32;
+    const float *d = (nl & 1) ? L.db32 : L.da32;
     const int ij = j * L.npx + i;
     if (field == 2)
-        phi[ij] = nsw > 0 ? (double)d[ij] : 0.0;
+        phi[ij] = nl > 0 ? (double)d[ij] : 0.0;
     else
         phi[ij] = phi[ij] + (double)d[ij];
 }
@@ -267,15 +325,11 @@ __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int fi
 
 void mgf_launch_reduce_err(const DevGroup &G, int record, hipStream_t st);
 
-static inline dim3 grid3(int ni, int nj, int nz, int by = BY)
-{
-    return dim3((unsigned)((ni + BX - 1) / BX), (unsigned)((nj + by - 1) / by), (unsigned)nz);
-}
 
 void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st)
 {
     if (field == 2) {
-        hipLaunchKernelGGL(k_asm_p_mx, grid3(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st,
+        hipLaunchKernelGGL(k_asm_p_mx, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st,
                            G.d);
         mgf_launch_reduce_err(G, record, st);
         return;
@@ -284,47 +338,122 @@ void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t 
     if (field == 0) ni = G.npx - 3;
     if (field == 1) nj = G.npy - 3;
     if (ni < 1 || nj < 1) return;
-    hipLaunchKernelGGL(k_asm_mx, grid3(ni, nj, G.count), dim3(BX, BY), 0, st, G.d, field);
+    hipLaunchKernelGGL(k_asm_mx, tile_grid(ni, nj, G.count), dim3(BX, BY), 0, st, G.d, field);
 }
 
-void mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
+template <int S, int MODE>
+static void launch_rbx(const DevGroup &G, int field, int launch, int last, int nx, int ny, int ty,
+                       hipStream_t st)
 {
-    int nx, ny;
+    if (ty == 32)
+        hipLaunchKernelGGL((k_rb32x<S, 32, MODE>), tile_grid(nx - 1, ny - 1, G.count, 32),
+                           dim3(BX, BY), 0, st, G.d, field, launch, last);
+    else
+        hipLaunchKernelGGL((k_rb32x<S, 16, MODE>), tile_grid(nx - 1, ny - 1, G.count, 16),
+                           dim3(BX, BY), 0, st, G.d, field, launch, last);
+}
+
+template <int MODE>
+static void launch_rbx_s(const DevGroup &G, int field, int launch, int last, int n, int nx, int ny,
+                         int ty, hipStream_t st)
+{
+    switch (n) {
+    case 1: launch_rbx<1, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    case 2: launch_rbx<2, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    case 3: launch_rbx<3, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    case 4: launch_rbx<4, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    case 5: launch_rbx<5, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    default: launch_rbx<6, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    }
+}
+
+static void sweep_extents(const DevGroup &G, int field, int &nx, int &ny)
+{
     switch (field) {
     case 0: nx = G.npx - 2; ny = G.npy - 1; break;
     case 1: nx = G.npx - 1; ny = G.npy - 2; break;
     default: nx = G.npx - 1; ny = G.npy - 1; break;
     }
-    if (nx - 1 < 1 || ny - 1 < 1 || (isolve != 1 && isolve != 2)) return;
-    for (int it = 0; it < nsw; ++it) {
-        if (isolve == 2)
-            hipLaunchKernelGGL(k_rb32, grid3(nx - 1, ny - 1, G.count, RB_TY), dim3(BX, BY), 0, st,
-                               G.d, field, it);
-        else
-            hipLaunchKernelGGL(k_jac32, grid3(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
-                               field, it);
-    }
 }
 
-void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
+SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int tile_rows, int split_assembly)
+{
+    SweepCfg c;
+    c.spl = sweeps_per_launch <= 0 ? MGF_AUTO_SWEEPS_PER_LAUNCH : sweeps_per_launch;
+    if (c.spl > MGF_MAX_SWEEPS_PER_LAUNCH) c.spl = MGF_MAX_SWEEPS_PER_LAUNCH;
+    c.ty = tile_rows == 32 ? 32 : tile_rows == 16 ? 16 : MGF_AUTO_SWEEP_TILE_ROWS;
+    c.fuse = split_assembly ? 0 : 1;
+    return c;
+}
+
+/* nsw sweeps on the stored FP32 equation, as launches of at most spl sweeps
+ * (red-black) or one sweep each (Jacobi); returns the number of launches,
+ * whose parity says which buffer holds the result */
+int mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
+                            hipStream_t st)
 {
     int nx, ny;
-    switch (field) {
-    case 0: nx = G.npx - 2; ny = G.npy - 1; break;
-    case 1: nx = G.npx - 1; ny = G.npy - 2; break;
-    default: nx = G.npx - 1; ny = G.npy - 1; break;
+    sweep_extents(G, field, nx, ny);
+    if (nx - 1 < 1 || ny - 1 < 1 || (isolve != 1 && isolve != 2)) return 0;
+    int launch = 0;
+    if (isolve == 1) {
+        for (; launch < nsw; ++launch)
+            hipLaunchKernelGGL(k_jac32, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                               G.d, field, launch);
+        return launch;
     }
-    if (nx - 1 < 1 || ny - 1 < 1 || (isolve != 1 && isolve != 2)) return;
-    for (int it = 0; it < nsw; ++it) {
-        if (isolve == 2)
-            hipLaunchKernelGGL(k_rb32, grid3(nx - 1, ny - 1, G.count, RB_TY), dim3(BX, BY), 0, st,
-                               G.d, field, it);
+    for (int done = 0; done < nsw; ++launch) {
+        const int n = nsw - done < cfg.spl ? nsw - done : cfg.spl;
+        launch_rbx_s<MODE_LOAD>(G, field, launch, 0, n, nx, ny, cfg.ty, st);
+        done += n;
+    }
+    return launch;
+}
+
+/* smoothing of one field in a relaxation: assembly, nsw sweeps and the
+ * update of the FP64 unknown (field 0 u, 1 v, 2 p', 3 sc).  Red-black with
+ * cfg.fuse: the first sweep launch assembles the equation itself, and for p'
+ * the last launch writes pp; otherwise (Jacobi, no sweeps, or split
+ * assembly) the assembly kernel stores the FP32 equation first.  The level's
+ * sum |div| is reduced after the p' assembly (history entry when record). */
+void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int nsw,
+                                  SweepCfg cfg, int record, hipStream_t st)
+{
+    int nx, ny;
+    sweep_extents(G, field, nx, ny);
+    if (isolve != 2 || nsw < 1 || !cfg.fuse || nx - 1 < 1 || ny - 1 < 1) {
+        mgf_launch_asm_mixed(G, field, record, st);
+        mgf_launch_smooth_mixed(G, field, isolve, nsw, cfg, st);
+        return;
+    }
+    int launch = 0;
+    for (int done = 0; done < nsw; ++launch) {
+        const int n = nsw - done < cfg.spl ? nsw - done : cfg.spl;
+        const int last = done + n == nsw;
+        if (launch > 0)
+            launch_rbx_s<MODE_LOAD>(G, field, launch, last, n, nx, ny, cfg.ty, st);
+        else if (last)
+            launch_rbx_s<MODE_ASM>(G, field, launch, last, n, nx, ny, cfg.ty, st);
         else
-            hipLaunchKernelGGL(k_jac32, grid3(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
-                               field, it);
+            launch_rbx_s<MODE_ASM_STORE>(G, field, launch, last, n, nx, ny, cfg.ty, st);
+        done += n;
     }
+    if (field == 2)
+        mgf_launch_reduce_err(G, record, st);
+    else
+        hipLaunchKernelGGL(k_apply_mx, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                           G.d, field, launch);
+}
+
+void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
+                             hipStream_t st)
+{
+    int nx, ny;
+    sweep_extents(G, field, nx, ny);
+    if (nx - 1 < 1 || ny - 1 < 1 || (isolve != 1 && isolve != 2)) return;
+    const int nl = mgf_launch_sweeps_mixed(G, field, isolve, nsw, cfg, st);
     /* without sweeps the correction is zero: u, v, sc stay, pp becomes 0 */
-    if (nsw > 0 || field == 2)
-        hipLaunchKernelGGL(k_apply_mx, grid3(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
-                           field, nsw);
+    if (nl > 0 || field == 2)
+        hipLaunchKernelGGL(k_apply_mx, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                           G.d, field, nl);
 }

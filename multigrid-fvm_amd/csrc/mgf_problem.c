@@ -32,6 +32,9 @@ void mgf_options_default(mgf_options *o)
     o->use_graph = 1;
     o->device = -1;
     o->strict_limits = 0;
+    o->sweeps_per_launch = 0;
+    o->sweep_tile_rows = 0;
+    o->split_assembly = 0;
 }
 
 static int err(int code, const char *fmt, const char *what)

@@ -526,22 +526,19 @@ extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
 static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg)
 {
     hipStream_t st = s->st;
+    const SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_tile_rows,
+                                       s->opt.split_assembly);
     mgf_launch_fluxes(G, st);
-    mgf_launch_asm_mixed(G, 0, 0, st);
-    mgf_launch_smooth_mixed(G, 0, s->isolve, s->nswpm, st);
-    mgf_launch_asm_mixed(G, 1, 0, st);
-    mgf_launch_smooth_mixed(G, 1, s->isolve, s->nswpm, st);
-    mgf_launch_asm_mixed(G, 2, leg == 0, st);
-    mgf_launch_smooth_mixed(G, 2, s->isolve, s->nswpp, st);
+    mgf_launch_relax_field_mixed(G, 0, s->isolve, s->nswpm, cfg, 0, st);
+    mgf_launch_relax_field_mixed(G, 1, s->isolve, s->nswpm, cfg, 0, st);
+    mgf_launch_relax_field_mixed(G, 2, s->isolve, s->nswpp, cfg, leg == 0, st);
     mgf_launch_update(G, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) {
         mgf_launch_outflow(G, st);
-        mgf_launch_asm_mixed(G, 3, 0, st);
-        mgf_launch_smooth_mixed(G, 3, s->isolve, s->nswpm, st);
+        mgf_launch_relax_field_mixed(G, 3, s->isolve, s->nswpm, cfg, 0, st);
     } else {
-        mgf_launch_asm_mixed(G, 3, 0, st);
-        mgf_launch_smooth_mixed(G, 3, s->isolve, s->nswpm, st);
+        mgf_launch_relax_field_mixed(G, 3, s->isolve, s->nswpm, cfg, 0, st);
         mgf_launch_outflow(G, st);
     }
 }
@@ -932,11 +929,17 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     for (int b = 0; b < s->nb; ++b) s->h_ctl[b].done = 0;
     (void)hipMemcpyAsync(s->d_ctl, s->h_ctl, (size_t)s->nb * sizeof(DevCtl), hipMemcpyHostToDevice,
                          s->st);
-    auto launch = [&](int nsw) {
+    /* mixed red-black: launches as a relaxation issues them, each of
+     * spl = min(sweeps per launch, sweeps of that field) sweeps */
+    const int nsw_field = field == 2 ? s->nswpp : s->nswpm;
+    SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_tile_rows, 1);
+    if (nsw_field >= 1 && cfg.spl > nsw_field) cfg.spl = nsw_field;
+    const int spl = (mixed && isolve == 2) ? cfg.spl : 1;
+    auto launch = [&](int nl) {
         if (mixed)
-            mgf_launch_sweeps_mixed(G, field, isolve, nsw, s->st);
+            mgf_launch_sweeps_mixed(G, field, isolve, nl * spl, cfg, s->st);
         else
-            mgf_launch_smooth(G, field, isolve, nsw, s->st);
+            mgf_launch_smooth(G, field, isolve, nl, s->st);
     };
     launch(2);                                            /* warm up */
     hipEvent_t e0, e1;
@@ -956,12 +959,12 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
         (void)hipFree(save);
     }
     if ((rc = check_launch(s))) return rc;
-    const int nx = field == 2 ? G.npx - 1 : G.npx - 2, ny = G.npy - 1;
+    int nx = field == 2 ? G.npx - 1 : G.npx - 2, ny = G.npy - 1;
     const double nunk = (double)(nx - 1) * (double)(ny - 1) * (double)G.count;
-    /* one launch per sweep; algorithmic words per unknown: aw ae as an ap su
-     * and phi read + write (FP64), or aw ae as an 1/ap r and d read + write
-     * (FP32) */
+    /* algorithmic words per unknown and sweep: aw ae as an ap su and phi read
+     * + write (FP64), or aw ae as an 1/ap r and d read + write (FP32); a
+     * launch counts its sweeps */
     if (mean_us) *mean_us = 1000.0 * (double)ms / (double)even;
-    if (bytes_per_launch) *bytes_per_launch = 8.0 * (mixed ? 4.0 : 8.0) * nunk;
+    if (bytes_per_launch) *bytes_per_launch = (double)spl * 8.0 * (mixed ? 4.0 : 8.0) * nunk;
     return MGF_OK;
 }

@@ -36,26 +36,64 @@ struct FluxInl {
     __device__ double Y(int k) const { return (0.5 * v[k]) * ay; }
 };
 
-/* descriptor of this workgroup's level, and the early exit once its
- * instance's stopping test has fired */
+/* Workgroups of a launch go to the 8 XCDs round-robin by linear id, and
+ * each XCD has its own L2.  2-D launches pad their rows of tiles to a
+ * multiple of 8 (tile_grid: gridDim.y = 8 b), and tile_id() hands XCD k the
+ * rows [k b, (k+1) b) of every group member: tiles whose halos overlap (same
+ * row, or neighbouring rows of one band) share an L2, and every member is
+ * spread over all XCDs, so members that are still iterating after others
+ * have stopped keep the whole chip busy.  Padding rows lie past the level's
+ * extent, where the kernels' range checks skip them.  Grids whose gridDim.y
+ * is not a multiple of 8 keep the hardware order. */
+#define MGF_NXCD 8u
+struct TileId {
+    int x, y, z;
+};
+__device__ __forceinline__ TileId tile_id()
+{
+    if (gridDim.y % MGF_NXCD) return TileId{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+    const unsigned gx = gridDim.x, b = gridDim.y / MGF_NXCD;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned k = lin % MGF_NXCD, m = lin / MGF_NXCD, B = b * gx;
+    const unsigned z = m / B, o = m - z * B, row = o / gx;
+    return TileId{(int)(o - row * gx), (int)(k * b + row), (int)z};
+}
+
+/* launch grid for ni x nj points per member in tiles of BX x by, rows of
+ * tiles padded to a multiple of 8 */
+static inline dim3 tile_grid(int ni, int nj, int nz, int by = BY)
+{
+    const unsigned gy = (unsigned)((nj + by - 1) / by);
+    return dim3((unsigned)((ni + BX - 1) / BX), (gy + MGF_NXCD - 1) / MGF_NXCD * MGF_NXCD,
+                (unsigned)nz);
+}
+
+/* descriptor of this workgroup's level (T = its logical tile), and the
+ * early exit once its instance's stopping test has fired */
 #define LEVEL(Ls)                                 \
+    const TileId T = tile_id();                   \
+    const DevLevel &L = (Ls)[T.z];                \
+    if (L.ctl->done) return;
+/* the same for the 1-D boundary kernels (no tiles, no renumbering) */
+#define LEVEL1D(Ls)                               \
     const DevLevel &L = (Ls)[blockIdx.z];         \
     if (L.ctl->done) return;
 #define PAIR(Fs, Cs)                              \
-    const DevLevel &F = (Fs)[blockIdx.z];         \
-    const DevLevel &C = (Cs)[blockIdx.z];         \
+    const TileId T = tile_id();                   \
+    const DevLevel &F = (Fs)[T.z];                \
+    const DevLevel &C = (Cs)[T.z];                \
     if (C.ctl->done) return;
 
-/* coefu + no-slip wall doubling at one u face (V6:825-858) */
-template <class FX>
-__device__ __forceinline__ Coef coefu_at(const DevLevel &L, const FX &f, int i, int j)
+/* coefu + no-slip wall doubling at one u face in row j (V6:825-858) from
+ * the face fluxes around it: cx at (i-1, i, i+1, row j), cy at (i, i+1) of
+ * rows j-1 and j */
+__device__ __forceinline__ Coef coefu_vals(const DevLevel &L, int j, double xw, double x0,
+                                           double xe, double ys0, double ys1, double y0, double y1)
 {
-    const int nx = L.npx;
-    const int ij = j * nx + i, ijs = ij - nx;
-    double cxm = 0.5 * (f.X(ij) + f.X(ij - 1));
-    double cxp = 0.5 * (f.X(ij) + f.X(ij + 1));
-    double cym = 0.5 * (f.Y(ijs) + f.Y(ijs + 1));
-    double cyp = 0.5 * (f.Y(ij) + f.Y(ij + 1));
+    double cxm = 0.5 * (x0 + xw);
+    double cxp = 0.5 * (x0 + xe);
+    double cym = 0.5 * (ys0 + ys1);
+    double cyp = 0.5 * (y0 + y1);
     Coef c;
     c.aw = fmax(fabs(cxm), L.du_x) + cxm;
     c.ae = fmax(fabs(cxp), L.du_x) - cxp;
@@ -67,16 +105,25 @@ __device__ __forceinline__ Coef coefu_at(const DevLevel &L, const FX &f, int i, 
     return c;
 }
 
-/* coefv + side-wall adds at one v face (V6:976-1015) */
 template <class FX>
-__device__ __forceinline__ Coef coefv_at(const DevLevel &L, const FX &f, int i, int j)
+__device__ __forceinline__ Coef coefu_at(const DevLevel &L, const FX &f, int i, int j)
 {
     const int nx = L.npx;
-    const int ij = j * nx + i, ijn = ij + nx, ijs = ij - nx;
-    double cxm = 0.5 * (f.X(ij - 1) + f.X(ijn - 1));
-    double cxp = 0.5 * (f.X(ij) + f.X(ijn));
-    double cym = 0.5 * (f.Y(ij) + f.Y(ijs));
-    double cyp = 0.5 * (f.Y(ij) + f.Y(ijn));
+    const int ij = j * nx + i, ijs = ij - nx;
+    return coefu_vals(L, j, f.X(ij - 1), f.X(ij), f.X(ij + 1), f.Y(ijs), f.Y(ijs + 1), f.Y(ij),
+                      f.Y(ij + 1));
+}
+
+/* coefv + side-wall adds at one v face in column i (V6:976-1015) from cx
+ * at (i-1, i) of rows j and j+1 and cy at rows j-1, j, j+1 of column i */
+__device__ __forceinline__ Coef coefv_vals(const DevLevel &L, int i, double xw, double xwn,
+                                           double x0, double xn, double ys, double y0, double yn)
+{
+    const int nx = L.npx;
+    double cxm = 0.5 * (xw + xwn);
+    double cxp = 0.5 * (x0 + xn);
+    double cym = 0.5 * (y0 + ys);
+    double cyp = 0.5 * (y0 + yn);
     Coef c;
     c.aw = fmax(fabs(cxm), L.du_x) + cxm;
     c.ae = fmax(fabs(cxp), L.du_x) - cxp;
@@ -88,15 +135,21 @@ __device__ __forceinline__ Coef coefv_at(const DevLevel &L, const FX &f, int i, 
     return c;
 }
 
-/* coefsc at one cell including the obstacle diffusion adds that the
- * reference applies on the host in obstacle-list order (V6:872-963) */
 template <class FX>
-__device__ __forceinline__ Coef coefsc_at(const DevLevel &L, const FX &f, int i, int j,
-                                          uint8_t fl)
+__device__ __forceinline__ Coef coefv_at(const DevLevel &L, const FX &f, int i, int j)
 {
     const int nx = L.npx;
-    const int ij = j * nx + i, ijs = ij - nx;
-    double cxm = f.X(ij - 1), cxp = f.X(ij), cym = f.Y(ijs), cyp = f.Y(ij);
+    const int ij = j * nx + i, ijn = ij + nx, ijs = ij - nx;
+    return coefv_vals(L, i, f.X(ij - 1), f.X(ijn - 1), f.X(ij), f.X(ijn), f.Y(ijs), f.Y(ij),
+                      f.Y(ijn));
+}
+
+/* coefsc at cell ij including the obstacle diffusion adds that the
+ * reference applies on the host in obstacle-list order (V6:872-963), from
+ * the fluxes through its four faces */
+__device__ __forceinline__ Coef coefsc_vals(const DevLevel &L, int ij, uint8_t fl, double cxm,
+                                            double cxp, double cym, double cyp)
+{
     Coef c;
     c.aw = fmax(fabs(cxm), L.ds_x) + cxm;
     c.ae = fmax(fabs(cxp), L.ds_x) - cxp;
@@ -118,9 +171,18 @@ __device__ __forceinline__ Coef coefsc_at(const DevLevel &L, const FX &f, int i,
     return c;
 }
 
+template <class FX>
+__device__ __forceinline__ Coef coefsc_at(const DevLevel &L, const FX &f, int i, int j,
+                                          uint8_t fl)
+{
+    const int nx = L.npx;
+    const int ij = j * nx + i, ijs = ij - nx;
+    return coefsc_vals(L, ij, fl, f.X(ij - 1), f.X(ij), f.Y(ijs), f.Y(ij));
+}
+
 #define RANGE2D(i0, i1, j0, j1)                                   \
-    const int i = (i0) + (int)(blockIdx.x * BX + threadIdx.x);    \
-    const int j = (j0) + (int)(blockIdx.y * BY + threadIdx.y);    \
+    const int i = (i0) + (int)(T.x * BX + threadIdx.x);           \
+    const int j = (j0) + (int)(T.y * BY + threadIdx.y);           \
     if (i > (i1) || j > (j1)) return;
 
 static inline dim3 grid2d(int ni, int nj)

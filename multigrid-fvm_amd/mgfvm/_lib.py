@@ -40,7 +40,9 @@ class MgfError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("device", ctypes.c_int), ("strict_limits", ctypes.c_int)]
+                ("device", ctypes.c_int), ("strict_limits", ctypes.c_int),
+                ("sweeps_per_launch", ctypes.c_int), ("sweep_tile_rows", ctypes.c_int),
+                ("split_assembly", ctypes.c_int)]
 
 
 class ProblemStruct(ctypes.Structure):
@@ -133,15 +135,19 @@ class Solver:
     """One problem resident on one GPU."""
 
     def __init__(self, inputdata: str, pattern: str, precision: int = PREC_FP64,
-                 use_graph: bool = True, device: int = -1, strict_limits: bool = False):
+                 use_graph: bool = True, device: int = -1, strict_limits: bool = False,
+                 sweeps_per_launch: int = 0, sweep_tile_rows: int = 0,
+                 split_assembly: bool = False):
         L = lib()
         prob = read_problem(inputdata, pattern)
         try:
-            self._init_from(prob, precision, use_graph, device, strict_limits)
+            self._init_from(prob, precision, use_graph, device, strict_limits, sweeps_per_launch,
+                            sweep_tile_rows, split_assembly)
         finally:
             free_problem(prob)
 
-    def _init_from(self, prob, precision, use_graph, device, strict_limits):
+    def _init_from(self, prob, precision, use_graph, device, strict_limits, sweeps_per_launch=0,
+                   sweep_tile_rows=0, split_assembly=False):
         L = lib()
         opt = Options()
         L.mgf_options_default(ctypes.byref(opt))
@@ -149,6 +155,9 @@ class Solver:
         opt.use_graph = 1 if use_graph else 0
         opt.device = device
         opt.strict_limits = 1 if strict_limits else 0
+        opt.sweeps_per_launch = sweeps_per_launch
+        opt.sweep_tile_rows = sweep_tile_rows
+        opt.split_assembly = 1 if split_assembly else 0
         h = ctypes.c_void_p()
         rc = L.mgf_create(prob, ctypes.byref(opt), ctypes.byref(h))
         if rc != 0:
@@ -240,7 +249,8 @@ class Batch:
     """Many independent problems solved together on one GPU."""
 
     def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
-                 use_graph: bool = True, device: int = -1):
+                 use_graph: bool = True, device: int = -1, sweeps_per_launch: int = 0,
+                 sweep_tile_rows: int = 0, split_assembly: bool = False):
         L = lib()
         probs = [read_problem(a, b) for a, b in pairs]
         try:
@@ -250,6 +260,9 @@ class Batch:
             opt.precision = precision
             opt.use_graph = 1 if use_graph else 0
             opt.device = device
+            opt.sweeps_per_launch = sweeps_per_launch
+            opt.sweep_tile_rows = sweep_tile_rows
+            opt.split_assembly = 1 if split_assembly else 0
             h = ctypes.c_void_p()
             rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))
             if rc != 0:

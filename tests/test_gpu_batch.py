@@ -136,3 +136,32 @@ def test_mixed_batch_matches_mixed_singles(tmp_path):
         for name in ("u", "v", "p", "sc"):
             assert np.array_equal(s.get(0, name), bt.get(k, 0, name)), (k, name)
         s.close()
+
+
+@pytest.mark.parametrize("spl,rows,split", [(2, 16, False), (3, 16, False), (3, 32, False),
+                                            (6, 32, False), (1, 16, False), (3, 16, True)])
+def test_sweep_launch_shapes_bit_identical(tmp_path, spl, rows, split):
+    """The launch shape of the FP32 red-black smoothing is result-neutral:
+    several sweeps per launch (temporal blocking, a halo of two points per
+    sweep), the tile height, and assembling the equation inside the first
+    sweep launch (halo recomputed per workgroup, sum |div| partials in the
+    assembly kernel's order) all give every member's history and fields bit
+    for bit as one sweep per launch after a separate assembly kernel."""
+    import mgfvm
+    probs = [p.replace(nswpp=6) for p in _mixed_problems()]
+    pairs = [p.write(str(tmp_path / f"b{k}")) for k, p in enumerate(probs)]
+    _say(f"[gpu] sweep launch shape spl={spl} rows={rows} split={split}")
+    base = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, sweeps_per_launch=1,
+                       split_assembly=True)
+    nb, cb = base.solve()
+    bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, sweeps_per_launch=spl,
+                     sweep_tile_rows=rows, split_assembly=split)
+    ncyc, conv = bt.solve()
+    assert (ncyc, conv) == (nb, cb)
+    for k in range(len(probs)):
+        assert np.array_equal(base.history(k), bt.history(k)), k
+        for name in ("u", "v", "p", "pp", "sc", "apu", "apv"):
+            assert np.array_equal(base.get(k, 0, name).view(np.uint64),
+                                  bt.get(k, 0, name).view(np.uint64)), (k, name)
+    base.close()
+    bt.close()
