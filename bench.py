@@ -17,13 +17,18 @@ algorithm does not converge (low Re on 512^2 with a 16^2 coarsest level,
 identical in the CPU restatement) run their full cycle limit and count as
 not converged.
 
-Also reported: the plain FP64 GPU path (one problem at a time, same
-problems, a sample of --plain-sample problems) and the ratio; a single
-1024^2 solve (64x64 random mask, Re 200, 5 levels) in FP64 and mixed; the
-roofline of the dominant kernel (one FP32 red-black sweep launch over the
-512^2 members, HIP events on the solver's stream); and a CPU baseline (the
-reference program built from its own source, one core, per-V-cycle time per
-grid size measured on bounded runs, times the cycle counts of the batch).
+Also reported: the plain FP64 GPU path (one problem at a time, the FP64
+kernels, no batching), estimated over all problems by stratified sampling
+(--plain-per-class solves per grid size and convergence class), and the
+ratio; a single 1024^2 solve (64x64 random mask, Re 200, 5 levels) in FP64
+and mixed; the roofline of the dominant kernel (the fused u relaxation
+launch over the 512^2 members, HIP events on the batch's stream); and a CPU
+baseline (the reference program built from its own source, one core,
+per-V-cycle time per grid size measured on bounded runs, times the cycle
+counts of the batch).
+
+--roofline-only runs just the dominant kernel's launches and a 1 GiB device
+copy, for the rocprofv3 counter passes that give `traffic`.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -93,12 +98,15 @@ def time_reference_cycle(prob, workdir, target_s=4.0):
     return (run(n) / n if n != probe else per), kind, n
 
 
-def sweep_roofline(probs, pairs, reps, cycles=None):
-    """Mean duration of one k_rb32 launch (a FP32 red-black sweep of the p'
-    equation over every member of the largest grid size, level 0), timed with
-    HIP events on the batch's own stream after that batch has run (so the
-    coefficients are the assembled ones).  Graph-replayed V-cycles cannot be
-    bracketed per kernel, so the launch is repeated on its own."""
+def kernel_roofline(probs, pairs, reps, cycles=None):
+    """The dominant kernel of a batched mixed V-cycle is the fused relaxation
+    launch of u / v / sc (k_rb32x<2,16,1>: FP64 assembly of the equation,
+    two FP32 red-black sweeps, FP32 correction out).  Its mean duration is
+    timed with HIP events on the batch's own stream for the u launch over
+    every member of the largest grid size at level 0, after that batch has
+    run (graph-replayed V-cycles cannot be bracketed per kernel, so the
+    launch is repeated on its own).  Also reported: the FP32 sweep kernel
+    (k_rb32x<3,16,0>, three p' sweeps on the stored equation)."""
     import mgfvm
     nmax = max(p.nfx for p in probs)
     big = [k for k in range(len(probs)) if probs[k].nfx == nmax]
@@ -109,15 +117,60 @@ def sweep_roofline(probs, pairs, reps, cycles=None):
             os.path.join(work, f"roof{k}")) for k in big]
     bb = mgfvm.Batch(big_pairs, precision=mgfvm.PREC_MIXED, device=0)
     bb.reset()
-    bb.solve()          # leaves assembled coefficients behind
-    us, nbytes = bb.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, reps)
+    bb.solve()          # leaves an assembled state behind
+    us, nbytes = bb.time_kernel(mgfvm.KERNEL_RELAX_U, 0, reps)
+    us_s, nbytes_s = bb.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, reps)
     bb.close()
     achieved = nbytes / (us * 1e-6) / 1e9
+    nunk = (nmax - 1) * nmax * len(big_pairs)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
-            "kernel": f"k_rb32: one FP32 red-black sweep of the p' equation over the "
-                      f"{len(big_pairs)} {nmax}^2 members (one launch)",
-            "mean_us": round(us, 2), "bytes_per_launch": nbytes}
+            "kernel": f"k_rb32x<2,16,1>: fused u relaxation launch (FP64 assembly + 2 FP32 "
+                      f"red-black sweeps) over the {len(big_pairs)} {nmax}^2 members, level 0",
+            "mean_us": round(us, 2), "bytes_per_launch": nbytes,
+            "bytes_per_unknown": round(nbytes / nunk, 2),
+            "sweep_kernel": {"kernel": "k_rb32x<3,16,0>: 3 FP32 red-black p' sweeps on the "
+                                       "stored equation, same members",
+                             "mean_us": round(us_s, 2), "bytes_per_launch": nbytes_s,
+                             "effective_GBps": round(nbytes_s / (us_s * 1e-6) / 1e9, 1)}}
+
+
+def plain_estimate(probs, pairs, ncyc, conv, per_class):
+    """Converged solves/s of the plain FP64 GPU path (one problem at a time,
+    the FP64 kernels, no batching) over the whole set, estimated by
+    stratified sampling: the problems are split by grid size and by whether
+    the batch converged them; per_class problems of each class are solved to
+    their stopping test, and each class's total time is its count times the
+    class's mean solve time.  Returns (value, detail)."""
+    import mgfvm
+    classes = {}
+    for k, p in enumerate(probs):
+        classes.setdefault((p.nfx, bool(conv[k])), []).append(k)
+    total_t, detail, mismatch = 0.0, [], 0
+    for key in sorted(classes):
+        ks = classes[key]
+        sample = ks[:per_class]
+        ts = []
+        for k in sample:
+            s = mgfvm.Solver(*pairs[k], device=0)
+            s.solve()
+            s.reset()
+            _sync(s)
+            t0 = time.perf_counter()
+            _, n, c = s.solve()
+            _sync(s)
+            ts.append(time.perf_counter() - t0)
+            mismatch += int(bool(c) != key[1])
+            s.close()
+        mean_t = sum(ts) / len(ts)
+        total_t += mean_t * len(ks)
+        detail.append(f"{key[0]}^2 {'converging' if key[1] else 'not converging'}: "
+                      f"{len(ks)} problems, {len(sample)} timed, {mean_t * 1e3:.1f} ms/solve")
+    nconv = sum(bool(c) for c in conv)
+    return (nconv / total_t if total_t > 0 else None), {
+        "method": "stratified by grid size and convergence; class time = count x mean of "
+                  "the timed solves", "classes": detail,
+        "sampled_convergence_mismatches": mismatch, "estimated_total_s": round(total_t, 2)}
 
 
 def roofline_only(args):
@@ -132,7 +185,7 @@ def roofline_only(args):
     try:
         probs = staggered_batch(args.count)
         pairs = [p.write(os.path.join(work, f"p{k}")) for k, p in enumerate(probs)]
-        roof = sweep_roofline(probs, pairs, args.reps, cycles=3)
+        roof = kernel_roofline(probs, pairs, args.reps, cycles=3)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         b.copy_(a)
@@ -149,12 +202,14 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=256, help="problems in the batch")
-    ap.add_argument("--plain-sample", type=int, default=32,
-                    help="problems solved one at a time for the plain FP64 GPU figure")
+    ap.add_argument("--plain-per-class", type=int, default=3,
+                    help="problems per class timed one at a time for the plain FP64 figure")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-single", action="store_true", help="skip the 1024^2 single solve")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-at-a-time figure")
-    ap.add_argument("--reps", type=int, default=100, help="sweep launches timed for the roofline")
+    ap.add_argument("--reps", type=int, default=100, help="launches timed for the roofline")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the batched V-cycles eagerly instead of replaying a hipGraph")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant kernel and a 1 GiB copy (for counter profiles)")
     args = ap.parse_args()
@@ -181,7 +236,8 @@ def main():
         pairs = [p.write(os.path.join(work, f"p{k}")) for k, p in enumerate(probs)]
 
         # ---- batched mixed precision (the measured line) ----------------
-        bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, device=0)
+        bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, device=0,
+                         use_graph=not args.no_graph)
         for _ in range(args.warmup):
             bt.reset()
             bt.solve()
@@ -199,28 +255,16 @@ def main():
         print(f"[bench] batched mixed: {conv_total} converged in {elapsed:.2f}s", file=sys.stderr,
               flush=True)
 
-        # ---- roofline: one FP32 sweep launch over the largest members ----
-        roof = sweep_roofline(probs, pairs, args.reps)
+        # ---- roofline of the dominant kernel ------------------------------
+        roof = kernel_roofline(probs, pairs, args.reps)
 
         # ---- plain FP64 GPU path: one problem at a time -----------------
-        plain_value, ks = None, []
+        plain_value, plain_detail = None, None
         if not args.no_plain:
-            ks = list(range(min(args.plain_sample, len(probs))))
-            plain_t, plain_conv = 0.0, 0
-            for k in ks:
-                s = mgfvm.Solver(*pairs[k], device=0)
-                s.solve()
-                s.reset()
-                _sync(s)
-                t0 = time.perf_counter()
-                _, n, c = s.solve()
-                _sync(s)
-                plain_t += time.perf_counter() - t0
-                plain_conv += int(c)
-                s.close()
-            plain_value = plain_conv / plain_t if plain_t > 0 else None
-            print(f"[bench] plain fp64: {plain_conv} converged in {plain_t:.2f}s", file=sys.stderr,
-                  flush=True)
+            plain_value, plain_detail = plain_estimate(probs, pairs, ncyc, conv,
+                                                       args.plain_per_class)
+            print(f"[bench] plain fp64: {plain_value:.3f} solves/s (stratified)",
+                  file=sys.stderr, flush=True)
 
         # ---- single large problem ----------------------------------------
         single = None
@@ -280,7 +324,7 @@ def main():
                        "v_cycles_max": max(ncyc),
                        "plain_fp64_gpu_solves_per_s":
                            round(plain_value, 3) if plain_value else None,
-                       "plain_sample_problems": len(ks),
+                       "plain_fp64_gpu_estimate": plain_detail,
                        "batched_mixed_vs_plain":
                            round(value / plain_value, 2) if plain_value else None,
                        "single_1024": single},

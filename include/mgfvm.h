@@ -92,8 +92,8 @@ typedef struct mgf_options {
     int sweeps_per_launch;     /* MGF_PREC_MIXED red-black: sweeps fused into one
                                   temporally blocked launch (1-6; 0: automatic).  The
                                   result does not depend on it. */
-    int sweep_tile_rows;       /* rows of those launches' tiles (16 or 32; 0: automatic);
-                                  result-neutral as well */
+    int sweep_threads;         /* threads per workgroup of those launches (256, 512 or
+                                  1024; 0: automatic); result-neutral as well */
     int split_assembly;        /* MGF_PREC_MIXED red-black: 1 = assemble each equation in
                                   a kernel of its own that stores it in FP32; 0 (default)
                                   = the first sweep launch assembles it on the fly.
@@ -199,6 +199,9 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
 /* Kernel ids for mgf_time_kernel */
 #define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation */
 #define MGF_KERNEL_SWEEP_U   1   /* one full smoother sweep of the u equation */
+#define MGF_KERNEL_RELAX_U   2   /* mixed red-black: the first launch of the u
+                                    relaxation (assembly + sweeps in one kernel) */
+#define MGF_KERNEL_RELAX_PP  3   /* the same for p' */
 /* Time `reps` smoother launches of level ig (for a batch: the launch
  * covering every problem's level at the depth of problem 0's level ig) with
  * HIP events on the solver's stream, on the current coefficients.  A launch
@@ -209,7 +212,11 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
  * duration of one launch in microseconds and the algorithmic bytes of its
  * sweeps (8 words per unknown and sweep: 4 neighbour coefficients,
  * diagonal, right-hand side, the unknown read and written; 8 bytes each in
- * FP64, 4 in FP32). */
+ * FP64, 4 in FP32).  MGF_KERNEL_RELAX_*: the launch that assembles the
+ * equation and runs the first min(sweeps_per_launch, nswpm or nswpp)
+ * sweeps; its bytes are the compulsory ones per unknown (the FP64 state the
+ * assembly reads, the flag byte, the FP32 correction or pp written, apu /
+ * apv, and the FP32 equation when later launches reuse it). */
 int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps,
                     double *mean_us, double *bytes_per_launch);
 

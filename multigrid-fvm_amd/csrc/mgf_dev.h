@@ -16,6 +16,7 @@
 #define MGF_DEV_H
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #define MGF_MAX_LEVELS 24
@@ -36,43 +37,63 @@ struct DevPhys {
     int ibc[4];
 };
 
+/* The level descriptor.  Its body is written once and instantiated twice:
+ * DevLevel with generic pointers (host side, and what the launches pass),
+ * and DevLevelG, the same layout with every array pointer in the global
+ * address space, which the kernels read the descriptor through.  Pointers
+ * loaded from memory are otherwise generic, and the compiler then emits
+ * flat loads and stores, which also count against the LDS wait counter: a
+ * kernel that mixes them with LDS traffic waits for its global loads at
+ * every LDS wait. */
+#define MGF_G __attribute__((address_space(1)))
+#define MGF_LEVEL_BODY(GQ)                                                              \
+    int npx, npy;                                                                     \
+    int ig;               /* level index within its instance (0 = finest) */          \
+    int npcopy;                                                                       \
+    int has_obs;                                                                      \
+    int ext;              /* level below the obstacle level: coarsened obstacles */   \
+    int nblk;             /* workgroups of the residual kernel (partials) */          \
+    int isolve;           /* 1 Jacobi, 2 red-black */                                 \
+    /* metrics, each computed on the host with the reference's expression */         \
+    double ax, ay;        /* areax (= dy), areay (= dx) */                            \
+    double du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, walls */   \
+    double ds_x, ds_y;    /* amusc*(areax/dx), amusc*(areay/dy): coefsc, its adds */  \
+    double axax, ayay;    /* areax*areax, areay*areay: coefp */                       \
+    double bodyx, bodyy;  /* (dpdxm*areax)*areay, (dpdym*areax)*areay */              \
+    DevPhys P;            /* the instance's physics / boundary types */               \
+    GQ DevCtl *ctl;       /* the instance's control block */                          \
+    /* state */                                                                       \
+    GQ double *u, *v, *p, *pp, *sc;                                                   \
+    GQ double *resu, *resv, *ressc;  /* FAS tau terms (zero on level 0) */            \
+    GQ double *apu, *apv;            /* momentum diagonals (persist; ghosts 1e30) */  \
+    /* scratch shared by the equations, as in the reference */                        \
+    GQ double *cx, *cy;                                                               \
+    GQ double *aw, *ae, *as, *an, *ap, *su;                                           \
+    GQ double *tmp;                  /* smoother ping-pong */                         \
+    GQ double *cu, *cv, *cp, *csc;   /* coarse-grid corrections (ghosts stay 0) */    \
+    GQ double *partial;              /* per-block partial sums of |div| */            \
+    /* mixed precision (null in FP64 mode): FP32 coefficients, 1/diagonal,           \
+     * right-hand side (the FP64 residual for u, v, sc; -div for p'), and the two    \
+     * buffers of the FP32 correction sweeps */                                       \
+    GQ float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32;                  \
+    /* obstacle data */                                                               \
+    GQ const uint8_t *flags;                                                          \
+    GQ const uint64_t *sccode;                                                        \
+    GQ const int *pslot;                                                              \
+    GQ const int *porig;                                                              \
+    GQ double *pstash;
+
 struct DevLevel {
-    int npx, npy;
-    int ig;               /* level index within its instance (0 = finest) */
-    int npcopy;
-    int has_obs;
-    int ext;              /* level below the obstacle level: coarsened obstacles */
-    int nblk;             /* workgroups of the residual kernel (partials) */
-    int isolve;           /* 1 Jacobi, 2 red-black */
-    /* metrics, each computed on the host with the reference's expression */
-    double ax, ay;        /* areax (= dy), areay (= dx) */
-    double du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, wall adds */
-    double ds_x, ds_y;    /* amusc*(areax/dx), amusc*(areay/dy): coefsc and its adds */
-    double axax, ayay;    /* areax*areax, areay*areay: coefp */
-    double bodyx, bodyy;  /* (dpdxm*areax)*areay, (dpdym*areax)*areay */
-    DevPhys P;            /* the instance's physics / boundary types */
-    DevCtl *ctl;          /* the instance's control block */
-    /* state */
-    double *u, *v, *p, *pp, *sc;
-    double *resu, *resv, *ressc;     /* FAS tau terms (zero on level 0) */
-    double *apu, *apv;               /* momentum diagonals (persist; ghosts 1e30) */
-    /* scratch shared by the equations, as in the reference */
-    double *cx, *cy;
-    double *aw, *ae, *as, *an, *ap, *su;
-    double *tmp;                     /* smoother ping-pong */
-    double *cu, *cv, *cp, *csc;      /* coarse-grid corrections (ghosts stay 0) */
-    double *partial;                 /* per-block partial sums of |div| */
-    /* mixed precision (null in FP64 mode): FP32 coefficients, 1/diagonal,
-     * right-hand side (the FP64 residual for u, v, sc; -div for p'), and
-     * the two buffers of the FP32 correction sweeps */
-    float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32;
-    /* obstacle data */
-    const uint8_t *flags;
-    const uint64_t *sccode;
-    const int *pslot;
-    const int *porig;
-    double *pstash;
+    MGF_LEVEL_BODY()
 };
+struct DevLevelG {
+    MGF_LEVEL_BODY(MGF_G)
+};
+static_assert(sizeof(DevLevel) == sizeof(DevLevelG), "descriptor views differ in layout");
+static_assert(offsetof(DevLevel, pstash) == offsetof(DevLevelG, pstash),
+              "descriptor views differ in layout");
+typedef MGF_G double gdbl;
+typedef MGF_G float gflt;
 
 /* a set of same-shape levels launched together */
 struct DevGroup {
@@ -108,15 +129,26 @@ void mgf_launch_prolong(const DevPair &R, hipStream_t st);
 void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
 void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st);
-/* shape of the temporally blocked red-black sweeps: sweeps per launch and
- * tile rows */
+/* shape of the temporally blocked red-black sweeps: sweeps per launch,
+ * threads per workgroup of the launches on the stored equation (nt) and of
+ * the assembling launches (nt_asm) -- each workgroup takes a 64 x
+ * MGF_SWEEP_TILE_ROWS tile -- and whether the assembly is fused into the
+ * first launch */
 struct SweepCfg {
-    int spl, ty, fuse;
+    int spl, nt, nt_asm, fuse;
 };
+#define MGF_SWEEP_TILE_ROWS 16
 #define MGF_AUTO_SWEEPS_PER_LAUNCH 3
 #define MGF_MAX_SWEEPS_PER_LAUNCH 6
-#define MGF_AUTO_SWEEP_TILE_ROWS 16
-SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int tile_rows, int split_assembly);
+/* sweeps of a launch that also assembles the equation */
+#define MGF_MAX_FUSED_SWEEPS 3
+#define MGF_AUTO_SWEEP_THREADS 512
+#define MGF_AUTO_ASM_THREADS 256
+SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int threads, int split_assembly);
+/* the first (assembling) launch of a fused relaxation alone, for timing;
+ * returns compulsory bytes per unknown */
+double mgf_launch_relax_first_mixed(const DevGroup &G, int field, int nsw, SweepCfg cfg,
+                                    int *n_out, hipStream_t st);
 /* assembly + sweeps + FP64 update of one field in a relaxation */
 void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int nsw,
                                   SweepCfg cfg, int record, hipStream_t st);

@@ -155,9 +155,9 @@ __global__ void __launch_bounds__(BX * BY) k_jacobi(const DevLevel *Ls, int fiel
 {
     LEVEL(Ls);
     int nx, ny;
-    double *base = field_of(L, field, nx, ny);
-    const double *src = parity ? L.tmp : base;
-    double *dst = parity ? base : L.tmp;
+    gdbl *base = field_of(L, field, nx, ny);
+    const gdbl *src = parity ? L.tmp : base;
+    gdbl *dst = parity ? base : L.tmp;
     RANGE2D(1, nx - 1, 1, ny - 1);
     const int npx = L.npx;
     const int ij = j * npx + i;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(BX * BY) k_copy_range(const DevLevel *Ls, int 
 {
     LEVEL(Ls);
     int nx, ny;
-    double *dst = field_of(L, field, nx, ny);
+    gdbl *dst = field_of(L, field, nx, ny);
     RANGE2D(1, nx - 1, 1, ny - 1);
     dst[j * L.npx + i] = L.tmp[j * L.npx + i];
 }
@@ -214,9 +214,9 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
 {
     LEVEL(Ls);
     int nx, ny;
-    const double *base = field_of(L, field, nx, ny);
-    const double *src = parity ? L.tmp : base;
-    double *dst = parity ? (double *)base : L.tmp;
+    const gdbl *base = field_of(L, field, nx, ny);
+    const gdbl *src = parity ? L.tmp : base;
+    gdbl *dst = parity ? (gdbl *)base : L.tmp;
     __shared__ double t[RB_TY + 4][BX + 4];
     const int npx = L.npx, npy = L.npy;
     const int i0 = 1 + T.x * BX, j0 = 1 + T.y * RB_TY;
@@ -421,12 +421,12 @@ __global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const Dev
 
 /* momentum residual at a u face with un-relaxed, un-penalised coefficients
  * (restru, V6:1496-1510 fine with resu, 1532-1545 coarse without) */
-__device__ __forceinline__ double rs_u(const DevLevel &L, int i, int j, double body, bool fine,
+__device__ __forceinline__ double rs_u(const DevLevelG &L, int i, int j, double body, bool fine,
                                        bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefu_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
-    const double *u = L.u;
+    const gdbl *u = L.u;
     double r = ((((c.aw * u[ij - 1] + c.ae * u[ij + 1]) + c.as * u[ij - nx]) + c.an * u[ij + nx]) -
                 c.ap * u[ij]) + L.ax * (L.p[ij] - L.p[ij + 1]);
     if (fine) r = r + L.resu[ij];
@@ -436,12 +436,12 @@ __device__ __forceinline__ double rs_u(const DevLevel &L, int i, int j, double b
     return r;
 }
 
-__device__ __forceinline__ double rs_v(const DevLevel &L, int i, int j, double body, bool fine,
+__device__ __forceinline__ double rs_v(const DevLevelG &L, int i, int j, double body, bool fine,
                                        bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefv_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
-    const double *v = L.v;
+    const gdbl *v = L.v;
     double r = ((((c.aw * v[ij - 1] + c.ae * v[ij + 1]) + c.as * v[ij - nx]) + c.an * v[ij + nx]) -
                 c.ap * v[ij]) + L.ay * (L.p[ij] - L.p[ij + nx]);
     if (fine) r = r + L.resv[ij];
@@ -451,12 +451,12 @@ __device__ __forceinline__ double rs_v(const DevLevel &L, int i, int j, double b
     return r;
 }
 
-__device__ __forceinline__ double rs_sc(const DevLevel &L, int i, int j, bool fine,
+__device__ __forceinline__ double rs_sc(const DevLevelG &L, int i, int j, bool fine,
                                         bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefsc_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j, L.flags[ij]);
-    const double *s = L.sc;
+    const gdbl *s = L.sc;
     double r = (((c.aw * s[ij - 1] + c.ae * s[ij + 1]) + c.as * s[ij - nx]) + c.an * s[ij + nx]) -
                c.ap * s[ij];
     if (fine) r = r + L.ressc[ij];
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const De
 /* face correction of prolu/prolv for coarse face k along the face direction,
  * mixing in the neighbour across it (3/4, 1/4), zero where the fine face at
  * (2ic, 2jc) is penalised (V6:1794-1798, 1870-1874) */
-__device__ __forceinline__ double corr_u(const DevLevel &F, const DevLevel &C, int ic, int jc,
+__device__ __forceinline__ double corr_u(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
                                          bool upper)
 {
     const int nxc = C.npx;
@@ -594,7 +594,7 @@ __device__ __forceinline__ double corr_u(const DevLevel &F, const DevLevel &C, i
     return c;
 }
 
-__device__ __forceinline__ double corr_v(const DevLevel &F, const DevLevel &C, int ic, int jc,
+__device__ __forceinline__ double corr_v(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
                                          bool right)
 {
     const int nxc = C.npx;

@@ -25,12 +25,12 @@ struct Pt32 {
     float aw, ae, as, an, iap, r;
 };
 
-__device__ __forceinline__ Pt32 load_pt(const DevLevel &L, int g)
+__device__ __forceinline__ Pt32 load_pt(const DevLevelG &L, int g)
 {
     return Pt32{L.aw32[g], L.ae32[g], L.as32[g], L.an32[g], L.iap32[g], L.r32[g]};
 }
 
-__device__ __forceinline__ void store_pt(const DevLevel &L, int g, const Pt32 &q)
+__device__ __forceinline__ void store_pt(const DevLevelG &L, int g, const Pt32 &q)
 {
     L.aw32[g] = q.aw;
     L.ae32[g] = q.ae;
@@ -44,14 +44,14 @@ __device__ __forceinline__ void store_pt(const DevLevel &L, int g, const Pt32 &q
  * to FP32 (field 0 u, 1 v, 3 sc); ap_out = the FP64 diagonal, which the
  * caller stores as apu / apv for u / v.  No global stores here, so a caller
  * assembling many points can keep all their loads in flight. */
-__device__ __forceinline__ Pt32 asm_pt(const DevLevel &L, int field, int i, int j, double &ap_out)
+__device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int j, double &ap_out)
 {
     const DevPhys &P = L.P;
     const int nx = L.npx;
     const int ij = j * nx + i;
     Coef c;
     double su, ap;
-    const double *phi;
+    const gdbl *phi;
     if (field == 0) {
         c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
         su = L.ax * (L.p[ij] - L.p[ij + 1]);
@@ -83,7 +83,7 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevel &L, int field, int i, int 
 
 /* masserror + coefp in FP64 (V6:1018-1125) at one cell, rounded to FP32;
  * mag = |div| */
-__device__ __forceinline__ Pt32 asm_pt_p(const DevLevel &L, int i, int j, double &mag)
+__device__ __forceinline__ Pt32 asm_pt_p(const DevLevelG &L, int i, int j, double &mag)
 {
     const DevPhys &P = L.P;
     const int nx = L.npx;
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p_mx(const DevLevel *Ls)
 }
 
 /* extents of the swept range of a field (the reference's solver arguments) */
-__device__ __forceinline__ void extents(const DevLevel &L, int field, int &nx, int &ny)
+__device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, int &ny)
 {
     (void)field_of(L, field, nx, ny);
 }
@@ -166,13 +166,17 @@ __device__ __forceinline__ void extents(const DevLevel &L, int field, int &nx, i
 #define MODE_LOAD 0
 #define MODE_ASM 1
 #define MODE_ASM_STORE 2
-template <int S, int TY, int MODE>
-__global__ void __launch_bounds__(BX * BY) k_rb32x(const DevLevel *Ls, int field, int launch,
-                                                   int last)
+template <int S, int NT, int MODE, int FIELD>
+__global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, int launch,
+                                              int last)
 {
     LEVEL(Ls);
+    /* FIELD >= 0: the field is fixed at compile time (assembling launches);
+     * -1: taken from field_rt */
+    const int field = FIELD >= 0 ? FIELD : field_rt;
+    constexpr int TY = MGF_SWEEP_TILE_ROWS;
     constexpr int H = 2 * S, RW = BX + 2 * H, RH = TY + 2 * H, NR = RW * RH;
-    constexpr int NT = BX * BY, NPT = (NR + NT - 1) / NT;
+    constexpr int NPT = (NR + NT - 1) / NT, NW = NT / BX;
     constexpr bool ASM = MODE != MODE_LOAD;
     constexpr int MR = ASM ? TY : 1;
     __shared__ float t[NR];
@@ -180,8 +184,8 @@ __global__ void __launch_bounds__(BX * BY) k_rb32x(const DevLevel *Ls, int field
     __shared__ double rows[MR];
     int nx, ny;
     extents(L, field, nx, ny);
-    const float *src = launch == 0 ? nullptr : ((launch & 1) ? L.db32 : L.da32);
-    float *dst = (launch & 1) ? L.da32 : L.db32;
+    const gflt *src = launch == 0 ? nullptr : ((launch & 1) ? L.db32 : L.da32);
+    gflt *dst = (launch & 1) ? L.da32 : L.db32;
     const bool pfield = field == 2;
     const int npx = L.npx;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * TY - H;
@@ -263,14 +267,15 @@ __global__ void __launch_bounds__(BX * BY) k_rb32x(const DevLevel *Ls, int field
         }
     }
     if (ASM && pfield) {
-        /* per 64 x 4 block of the tile: wave w sums row w of the block with
-         * block_sum's shuffle tree, then the four rows in order */
+        /* per 64 x 4 block of the tile: each row of the tile is summed by
+         * one wave with block_sum's shuffle tree, then the four rows of a
+         * block in order */
         const int w = threadIdx.y, lane = threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < TY / BY; ++q) {
-            double v = mg[q * BY + w][lane];
+        for (int row = w; row < TY; row += NW) {
+            double v = mg[row][lane];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0) rows[q * BY + w] = v;
+            if (lane == 0) rows[row] = v;
         }
         __syncthreads();
         if (tid < TY / BY) {
@@ -291,8 +296,8 @@ __global__ void __launch_bounds__(BX * BY) k_jac32(const DevLevel *Ls, int field
     int nx, ny;
     extents(L, field, nx, ny);
     RANGE2D(1, nx - 1, 1, ny - 1);
-    const float *src = it == 0 ? nullptr : ((it & 1) ? L.db32 : L.da32);
-    float *dst = (it & 1) ? L.da32 : L.db32;
+    const gflt *src = it == 0 ? nullptr : ((it & 1) ? L.db32 : L.da32);
+    gflt *dst = (it & 1) ? L.da32 : L.db32;
     const int npx = L.npx, ij = j * npx + i;
     float W = 0.f, E = 0.f, S = 0.f, N = 0.f;
     if (src) {
@@ -311,9 +316,9 @@ __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int fi
 {
     LEVEL(Ls);
     int nx, ny;
-    double *phi = field_of(L, field, nx, ny);
+    gdbl *phi = field_of(L, field, nx, ny);
     RANGE2D(1, nx - 1, 1, ny - 1);
-    const float *d = (nl & 1) ? L.db32 : L.da32;
+    const gflt *d = (nl & 1) ? L.db32 : L.da32;
     const int ij = j * L.npx + i;
     if (field == 2)
         phi[ij] = nl > 0 ? (double)d[ij] : 0.0;
@@ -341,29 +346,65 @@ void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t 
     hipLaunchKernelGGL(k_asm_mx, tile_grid(ni, nj, G.count), dim3(BX, BY), 0, st, G.d, field);
 }
 
-template <int S, int MODE>
-static void launch_rbx(const DevGroup &G, int field, int launch, int last, int nx, int ny, int ty,
-                       hipStream_t st)
+template <int S, int MODE, int FIELD>
+static void launch_rbx_f(const DevGroup &G, int field, int launch, int last, int nx, int ny,
+                         int nt, hipStream_t st)
 {
-    if (ty == 32)
-        hipLaunchKernelGGL((k_rb32x<S, 32, MODE>), tile_grid(nx - 1, ny - 1, G.count, 32),
-                           dim3(BX, BY), 0, st, G.d, field, launch, last);
+    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SWEEP_TILE_ROWS);
+    if (nt == 1024)
+        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE, FIELD>), g, dim3(BX, 16), 0, st, G.d, field,
+                           launch, last);
+    else if (nt == 512)
+        hipLaunchKernelGGL((k_rb32x<S, 512, MODE, FIELD>), g, dim3(BX, 8), 0, st, G.d, field,
+                           launch, last);
     else
-        hipLaunchKernelGGL((k_rb32x<S, 16, MODE>), tile_grid(nx - 1, ny - 1, G.count, 16),
-                           dim3(BX, BY), 0, st, G.d, field, launch, last);
+        hipLaunchKernelGGL((k_rb32x<S, 256, MODE, FIELD>), g, dim3(BX, 4), 0, st, G.d, field,
+                           launch, last);
 }
 
-template <int MODE>
-static void launch_rbx_s(const DevGroup &G, int field, int launch, int last, int n, int nx, int ny,
-                         int ty, hipStream_t st)
+/* sweeps on the stored equation (any field) */
+static void launch_load(const DevGroup &G, int field, int launch, int last, int n, int nx, int ny,
+                        int nt, hipStream_t st)
 {
     switch (n) {
-    case 1: launch_rbx<1, MODE>(G, field, launch, last, nx, ny, ty, st); break;
-    case 2: launch_rbx<2, MODE>(G, field, launch, last, nx, ny, ty, st); break;
-    case 3: launch_rbx<3, MODE>(G, field, launch, last, nx, ny, ty, st); break;
-    case 4: launch_rbx<4, MODE>(G, field, launch, last, nx, ny, ty, st); break;
-    case 5: launch_rbx<5, MODE>(G, field, launch, last, nx, ny, ty, st); break;
-    default: launch_rbx<6, MODE>(G, field, launch, last, nx, ny, ty, st); break;
+    case 1: launch_rbx_f<1, MODE_LOAD, -1>(G, field, launch, last, nx, ny, nt, st); break;
+    case 2: launch_rbx_f<2, MODE_LOAD, -1>(G, field, launch, last, nx, ny, nt, st); break;
+    case 3: launch_rbx_f<3, MODE_LOAD, -1>(G, field, launch, last, nx, ny, nt, st); break;
+    case 4: launch_rbx_f<4, MODE_LOAD, -1>(G, field, launch, last, nx, ny, nt, st); break;
+    case 5: launch_rbx_f<5, MODE_LOAD, -1>(G, field, launch, last, nx, ny, nt, st); break;
+    default: launch_rbx_f<6, MODE_LOAD, -1>(G, field, launch, last, nx, ny, nt, st); break;
+    }
+}
+
+template <int S, int MODE>
+static void launch_asm_s(const DevGroup &G, int field, int last, int nx, int ny, int nt,
+                         hipStream_t st)
+{
+    switch (field) {
+    case 0: launch_rbx_f<S, MODE, 0>(G, field, 0, last, nx, ny, nt, st); break;
+    case 1: launch_rbx_f<S, MODE, 1>(G, field, 0, last, nx, ny, nt, st); break;
+    case 2: launch_rbx_f<S, MODE, 2>(G, field, 0, last, nx, ny, nt, st); break;
+    default: launch_rbx_f<S, MODE, 3>(G, field, 0, last, nx, ny, nt, st); break;
+    }
+}
+
+/* the first, assembling launch of a relaxation: n <= MGF_MAX_FUSED_SWEEPS
+ * sweeps; store: later launches follow and need the stored equation */
+static void launch_asm(const DevGroup &G, int field, int last, bool store, int n, int nx, int ny,
+                       int nt, hipStream_t st)
+{
+    if (store) {
+        switch (n) {
+        case 1: launch_asm_s<1, MODE_ASM_STORE>(G, field, last, nx, ny, nt, st); break;
+        case 2: launch_asm_s<2, MODE_ASM_STORE>(G, field, last, nx, ny, nt, st); break;
+        default: launch_asm_s<3, MODE_ASM_STORE>(G, field, last, nx, ny, nt, st); break;
+        }
+    } else {
+        switch (n) {
+        case 1: launch_asm_s<1, MODE_ASM>(G, field, last, nx, ny, nt, st); break;
+        case 2: launch_asm_s<2, MODE_ASM>(G, field, last, nx, ny, nt, st); break;
+        default: launch_asm_s<3, MODE_ASM>(G, field, last, nx, ny, nt, st); break;
+        }
     }
 }
 
@@ -376,12 +417,14 @@ static void sweep_extents(const DevGroup &G, int field, int &nx, int &ny)
     }
 }
 
-SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int tile_rows, int split_assembly)
+SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int threads, int split_assembly)
 {
     SweepCfg c;
     c.spl = sweeps_per_launch <= 0 ? MGF_AUTO_SWEEPS_PER_LAUNCH : sweeps_per_launch;
     if (c.spl > MGF_MAX_SWEEPS_PER_LAUNCH) c.spl = MGF_MAX_SWEEPS_PER_LAUNCH;
-    c.ty = tile_rows == 32 ? 32 : tile_rows == 16 ? 16 : MGF_AUTO_SWEEP_TILE_ROWS;
+    const bool given = threads == 256 || threads == 512 || threads == 1024;
+    c.nt = given ? threads : MGF_AUTO_SWEEP_THREADS;
+    c.nt_asm = given ? threads : MGF_AUTO_ASM_THREADS;
     c.fuse = split_assembly ? 0 : 1;
     return c;
 }
@@ -404,7 +447,7 @@ int mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, S
     }
     for (int done = 0; done < nsw; ++launch) {
         const int n = nsw - done < cfg.spl ? nsw - done : cfg.spl;
-        launch_rbx_s<MODE_LOAD>(G, field, launch, 0, n, nx, ny, cfg.ty, st);
+        launch_load(G, field, launch, 0, n, nx, ny, cfg.nt, st);
         done += n;
     }
     return launch;
@@ -428,14 +471,14 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
     }
     int launch = 0;
     for (int done = 0; done < nsw; ++launch) {
-        const int n = nsw - done < cfg.spl ? nsw - done : cfg.spl;
+        int lim = cfg.spl;
+        if (launch == 0 && lim > MGF_MAX_FUSED_SWEEPS) lim = MGF_MAX_FUSED_SWEEPS;
+        const int n = nsw - done < lim ? nsw - done : lim;
         const int last = done + n == nsw;
         if (launch > 0)
-            launch_rbx_s<MODE_LOAD>(G, field, launch, last, n, nx, ny, cfg.ty, st);
-        else if (last)
-            launch_rbx_s<MODE_ASM>(G, field, launch, last, n, nx, ny, cfg.ty, st);
+            launch_load(G, field, launch, last, n, nx, ny, cfg.nt, st);
         else
-            launch_rbx_s<MODE_ASM_STORE>(G, field, launch, last, n, nx, ny, cfg.ty, st);
+            launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
         done += n;
     }
     if (field == 2)
@@ -443,6 +486,34 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
     else
         hipLaunchKernelGGL(k_apply_mx, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, field, launch);
+}
+
+/* the first launch of a fused red-black relaxation of a field on its own
+ * (for timing: it only rewrites what it wrote before, given unchanged
+ * inputs); *n_out = its sweeps.  Returns the compulsory HBM bytes per
+ * unknown: the FP64 inputs of the assembly read once (u: cx, cy, p, u, resu;
+ * v likewise; sc: cx, cy, sc, ressc; p': u, v, apu, apv), the flag byte, d
+ * written in FP32 (p' alone: pp in FP64), apu / apv written for u / v, and
+ * the FP32 equation (6 words) when later launches follow. */
+double mgf_launch_relax_first_mixed(const DevGroup &G, int field, int nsw, SweepCfg cfg,
+                                    int *n_out, hipStream_t st)
+{
+    int nx, ny;
+    sweep_extents(G, field, nx, ny);
+    int lim = cfg.spl < MGF_MAX_FUSED_SWEEPS ? cfg.spl : MGF_MAX_FUSED_SWEEPS;
+    const int n = nsw < lim ? nsw : lim;
+    const int last = n == nsw;
+    *n_out = n;
+    if (n < 1 || nx - 1 < 1 || ny - 1 < 1) return 0.0;
+    launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
+    double b = 1.0 + (last ? 0.0 : 24.0);
+    switch (field) {
+    case 0:
+    case 1: b += 5 * 8.0 + 4.0 + 8.0; break;
+    case 2: b += 4 * 8.0 + (last ? 8.0 : 4.0); break;
+    default: b += 4 * 8.0 + 4.0; break;
+    }
+    return b;
 }
 
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,

@@ -33,7 +33,7 @@ void mgf_options_default(mgf_options *o)
     o->device = -1;
     o->strict_limits = 0;
     o->sweeps_per_launch = 0;
-    o->sweep_tile_rows = 0;
+    o->sweep_threads = 0;
     o->split_assembly = 0;
 }
 

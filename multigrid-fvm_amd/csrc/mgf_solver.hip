@@ -526,7 +526,7 @@ extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
 static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg)
 {
     hipStream_t st = s->st;
-    const SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_tile_rows,
+    const SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads,
                                        s->opt.split_assembly);
     mgf_launch_fluxes(G, st);
     mgf_launch_relax_field_mixed(G, 0, s->isolve, s->nswpm, cfg, 0, st);
@@ -899,9 +899,13 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
 {
     if (!s || reps < 1) return set_err(s, MGF_ERR_ARG, "bad arguments");
     if (ig < 0 || ig >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad level");
-    if (kernel != MGF_KERNEL_SWEEP_PP && kernel != MGF_KERNEL_SWEEP_U)
+    if (kernel < MGF_KERNEL_SWEEP_PP || kernel > MGF_KERNEL_RELAX_PP)
         return set_err(s, MGF_ERR_ARG, "unknown kernel id");
     const int mixed = s->opt.precision == MGF_PREC_MIXED;
+    const int fused = kernel == MGF_KERNEL_RELAX_U || kernel == MGF_KERNEL_RELAX_PP;
+    if (fused && (!mixed || s->isolve != 2 || s->opt.split_assembly))
+        return set_err(s, MGF_ERR_UNSUPPORTED,
+                       "fused relaxation launches exist for mixed-precision red-black only");
     int rc;
     if (!mixed && (rc = single_only(s))) return rc;
     /* the group of every instance's level at instance 0's depth of level ig */
@@ -914,7 +918,7 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     }
     if (g < 0) return set_err(s, MGF_ERR_ARG, "no group for level %d", ig);
     const DevGroup &G = s->groups[g];
-    const int field = kernel == MGF_KERNEL_SWEEP_PP ? 2 : 0;
+    const int field = (kernel == MGF_KERNEL_SWEEP_PP || kernel == MGF_KERNEL_RELAX_PP) ? 2 : 0;
     const int isolve = s->isolve == 1 ? 1 : 2;
     const int even = reps + (reps & 1);    /* an even count ends without a copy-back */
     double *save = NULL, *phi = NULL;
@@ -932,14 +936,21 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     /* mixed red-black: launches as a relaxation issues them, each of
      * spl = min(sweeps per launch, sweeps of that field) sweeps */
     const int nsw_field = field == 2 ? s->nswpp : s->nswpm;
-    SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_tile_rows, 1);
+    SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads, 1);
     if (nsw_field >= 1 && cfg.spl > nsw_field) cfg.spl = nsw_field;
     const int spl = (mixed && isolve == 2) ? cfg.spl : 1;
+    const SweepCfg fcfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads, 0);
+    double fused_bpu = 0.0;
     auto launch = [&](int nl) {
-        if (mixed)
+        if (fused) {
+            int nf = 0;
+            for (int k = 0; k < nl; ++k)
+                fused_bpu = mgf_launch_relax_first_mixed(G, field, nsw_field, fcfg, &nf, s->st);
+        } else if (mixed) {
             mgf_launch_sweeps_mixed(G, field, isolve, nl * spl, cfg, s->st);
-        else
+        } else {
             mgf_launch_smooth(G, field, isolve, nl, s->st);
+        }
     };
     launch(2);                                            /* warm up */
     hipEvent_t e0, e1;
@@ -965,6 +976,7 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
      * + write (FP64), or aw ae as an 1/ap r and d read + write (FP32); a
      * launch counts its sweeps */
     if (mean_us) *mean_us = 1000.0 * (double)ms / (double)even;
-    if (bytes_per_launch) *bytes_per_launch = (double)spl * 8.0 * (mixed ? 4.0 : 8.0) * nunk;
+    if (bytes_per_launch)
+        *bytes_per_launch = fused ? fused_bpu * nunk : (double)spl * 8.0 * (mixed ? 4.0 : 8.0) * nunk;
     return MGF_OK;
 }

@@ -22,7 +22,7 @@ struct Coef {
 /* face fluxes from the materialised arrays (relaxation: cx/cy are computed
  * once at its start, V6:201, and the u/v/sc assemblies read those) */
 struct FluxArr {
-    const double *cx, *cy;
+    const gdbl *cx, *cy;
     __device__ double X(int k) const { return cx[k]; }
     __device__ double Y(int k) const { return cy[k]; }
 };
@@ -30,7 +30,7 @@ struct FluxArr {
  * reference recomputes them right before, V6:215-216, with nothing changing
  * u/v in between), same expression as fluxes (V6:690-691) */
 struct FluxInl {
-    const double *u, *v;
+    const gdbl *u, *v;
     double ax, ay;
     __device__ double X(int k) const { return (0.5 * u[k]) * ax; }
     __device__ double Y(int k) const { return (0.5 * v[k]) * ay; }
@@ -72,22 +72,22 @@ static inline dim3 tile_grid(int ni, int nj, int nz, int by = BY)
  * early exit once its instance's stopping test has fired */
 #define LEVEL(Ls)                                 \
     const TileId T = tile_id();                   \
-    const DevLevel &L = (Ls)[T.z];                \
+    const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[T.z]; \
     if (L.ctl->done) return;
 /* the same for the 1-D boundary kernels (no tiles, no renumbering) */
 #define LEVEL1D(Ls)                               \
-    const DevLevel &L = (Ls)[blockIdx.z];         \
+    const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[blockIdx.z]; \
     if (L.ctl->done) return;
 #define PAIR(Fs, Cs)                              \
     const TileId T = tile_id();                   \
-    const DevLevel &F = (Fs)[T.z];                \
-    const DevLevel &C = (Cs)[T.z];                \
+    const DevLevelG &F = reinterpret_cast<const DevLevelG *>(Fs)[T.z]; \
+    const DevLevelG &C = reinterpret_cast<const DevLevelG *>(Cs)[T.z]; \
     if (C.ctl->done) return;
 
 /* coefu + no-slip wall doubling at one u face in row j (V6:825-858) from
  * the face fluxes around it: cx at (i-1, i, i+1, row j), cy at (i, i+1) of
  * rows j-1 and j */
-__device__ __forceinline__ Coef coefu_vals(const DevLevel &L, int j, double xw, double x0,
+__device__ __forceinline__ Coef coefu_vals(const DevLevelG &L, int j, double xw, double x0,
                                            double xe, double ys0, double ys1, double y0, double y1)
 {
     double cxm = 0.5 * (x0 + xw);
@@ -106,7 +106,7 @@ __device__ __forceinline__ Coef coefu_vals(const DevLevel &L, int j, double xw, 
 }
 
 template <class FX>
-__device__ __forceinline__ Coef coefu_at(const DevLevel &L, const FX &f, int i, int j)
+__device__ __forceinline__ Coef coefu_at(const DevLevelG &L, const FX &f, int i, int j)
 {
     const int nx = L.npx;
     const int ij = j * nx + i, ijs = ij - nx;
@@ -116,7 +116,7 @@ __device__ __forceinline__ Coef coefu_at(const DevLevel &L, const FX &f, int i, 
 
 /* coefv + side-wall adds at one v face in column i (V6:976-1015) from cx
  * at (i-1, i) of rows j and j+1 and cy at rows j-1, j, j+1 of column i */
-__device__ __forceinline__ Coef coefv_vals(const DevLevel &L, int i, double xw, double xwn,
+__device__ __forceinline__ Coef coefv_vals(const DevLevelG &L, int i, double xw, double xwn,
                                            double x0, double xn, double ys, double y0, double yn)
 {
     const int nx = L.npx;
@@ -136,7 +136,7 @@ __device__ __forceinline__ Coef coefv_vals(const DevLevel &L, int i, double xw, 
 }
 
 template <class FX>
-__device__ __forceinline__ Coef coefv_at(const DevLevel &L, const FX &f, int i, int j)
+__device__ __forceinline__ Coef coefv_at(const DevLevelG &L, const FX &f, int i, int j)
 {
     const int nx = L.npx;
     const int ij = j * nx + i, ijn = ij + nx, ijs = ij - nx;
@@ -147,7 +147,7 @@ __device__ __forceinline__ Coef coefv_at(const DevLevel &L, const FX &f, int i, 
 /* coefsc at cell ij including the obstacle diffusion adds that the
  * reference applies on the host in obstacle-list order (V6:872-963), from
  * the fluxes through its four faces */
-__device__ __forceinline__ Coef coefsc_vals(const DevLevel &L, int ij, uint8_t fl, double cxm,
+__device__ __forceinline__ Coef coefsc_vals(const DevLevelG &L, int ij, uint8_t fl, double cxm,
                                             double cxp, double cym, double cyp)
 {
     Coef c;
@@ -172,7 +172,7 @@ __device__ __forceinline__ Coef coefsc_vals(const DevLevel &L, int ij, uint8_t f
 }
 
 template <class FX>
-__device__ __forceinline__ Coef coefsc_at(const DevLevel &L, const FX &f, int i, int j,
+__device__ __forceinline__ Coef coefsc_at(const DevLevelG &L, const FX &f, int i, int j,
                                           uint8_t fl)
 {
     const int nx = L.npx;
@@ -205,7 +205,7 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 
 /* the field a smoother works on and the reference's solver extents
  * (V6:701-736: u (npx-2, npy-1), v (npx-1, npy-2), pp and sc (npx-1, npy-1)) */
-__device__ __forceinline__ double *field_of(const DevLevel &L, int field, int &nx, int &ny)
+__device__ __forceinline__ gdbl *field_of(const DevLevelG &L, int field, int &nx, int &ny)
 {
     switch (field) {
     case 0: nx = L.npx - 2; ny = L.npy - 1; return L.u;

@@ -30,6 +30,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 KERNEL_SWEEP_PP = 0
 KERNEL_SWEEP_U = 1
+KERNEL_RELAX_U = 2
+KERNEL_RELAX_PP = 3
 
 
 class MgfError(RuntimeError):
@@ -41,7 +43,7 @@ class MgfError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("device", ctypes.c_int), ("strict_limits", ctypes.c_int),
-                ("sweeps_per_launch", ctypes.c_int), ("sweep_tile_rows", ctypes.c_int),
+                ("sweeps_per_launch", ctypes.c_int), ("sweep_threads", ctypes.c_int),
                 ("split_assembly", ctypes.c_int)]
 
 
@@ -136,18 +138,18 @@ class Solver:
 
     def __init__(self, inputdata: str, pattern: str, precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, strict_limits: bool = False,
-                 sweeps_per_launch: int = 0, sweep_tile_rows: int = 0,
+                 sweeps_per_launch: int = 0, sweep_threads: int = 0,
                  split_assembly: bool = False):
         L = lib()
         prob = read_problem(inputdata, pattern)
         try:
             self._init_from(prob, precision, use_graph, device, strict_limits, sweeps_per_launch,
-                            sweep_tile_rows, split_assembly)
+                            sweep_threads, split_assembly)
         finally:
             free_problem(prob)
 
     def _init_from(self, prob, precision, use_graph, device, strict_limits, sweeps_per_launch=0,
-                   sweep_tile_rows=0, split_assembly=False):
+                   sweep_threads=0, split_assembly=False):
         L = lib()
         opt = Options()
         L.mgf_options_default(ctypes.byref(opt))
@@ -156,7 +158,7 @@ class Solver:
         opt.device = device
         opt.strict_limits = 1 if strict_limits else 0
         opt.sweeps_per_launch = sweeps_per_launch
-        opt.sweep_tile_rows = sweep_tile_rows
+        opt.sweep_threads = sweep_threads
         opt.split_assembly = 1 if split_assembly else 0
         h = ctypes.c_void_p()
         rc = L.mgf_create(prob, ctypes.byref(opt), ctypes.byref(h))
@@ -250,7 +252,7 @@ class Batch:
 
     def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, sweeps_per_launch: int = 0,
-                 sweep_tile_rows: int = 0, split_assembly: bool = False):
+                 sweep_threads: int = 0, split_assembly: bool = False):
         L = lib()
         probs = [read_problem(a, b) for a, b in pairs]
         try:
@@ -261,7 +263,7 @@ class Batch:
             opt.use_graph = 1 if use_graph else 0
             opt.device = device
             opt.sweeps_per_launch = sweeps_per_launch
-            opt.sweep_tile_rows = sweep_tile_rows
+            opt.sweep_threads = sweep_threads
             opt.split_assembly = 1 if split_assembly else 0
             h = ctypes.c_void_p()
             rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))

@@ -138,9 +138,9 @@ def test_mixed_batch_matches_mixed_singles(tmp_path):
         s.close()
 
 
-@pytest.mark.parametrize("spl,rows,split", [(2, 16, False), (3, 16, False), (3, 32, False),
-                                            (6, 32, False), (1, 16, False), (3, 16, True)])
-def test_sweep_launch_shapes_bit_identical(tmp_path, spl, rows, split):
+@pytest.mark.parametrize("spl,threads,split", [(2, 256, False), (3, 256, False), (3, 512, False),
+                                               (6, 1024, False), (1, 256, False), (3, 512, True)])
+def test_sweep_launch_shapes_bit_identical(tmp_path, spl, threads, split):
     """The launch shape of the FP32 red-black smoothing is result-neutral:
     several sweeps per launch (temporal blocking, a halo of two points per
     sweep), the tile height, and assembling the equation inside the first
@@ -150,12 +150,12 @@ def test_sweep_launch_shapes_bit_identical(tmp_path, spl, rows, split):
     import mgfvm
     probs = [p.replace(nswpp=6) for p in _mixed_problems()]
     pairs = [p.write(str(tmp_path / f"b{k}")) for k, p in enumerate(probs)]
-    _say(f"[gpu] sweep launch shape spl={spl} rows={rows} split={split}")
+    _say(f"[gpu] sweep launch shape spl={spl} threads={threads} split={split}")
     base = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, sweeps_per_launch=1,
                        split_assembly=True)
     nb, cb = base.solve()
     bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, sweeps_per_launch=spl,
-                     sweep_tile_rows=rows, split_assembly=split)
+                     sweep_threads=threads, split_assembly=split)
     ncyc, conv = bt.solve()
     assert (ncyc, conv) == (nb, cb)
     for k in range(len(probs)):
