@@ -99,14 +99,14 @@ def time_reference_cycle(prob, workdir, target_s=4.0):
 
 
 def kernel_roofline(probs, pairs, reps, cycles=None):
-    """The dominant kernel of a batched mixed V-cycle is the fused relaxation
-    launch of u / v / sc (k_rb32x<2,16,1>: FP64 assembly of the equation,
-    two FP32 red-black sweeps, FP32 correction out).  Its mean duration is
+    """The dominant kernels of a batched mixed V-cycle are the fused
+    relaxation launches of u / v / sc (k_rb32x<2,256,1,field>: FP64 assembly
+    of the equation, two FP32 red-black sweeps, FP32 correction out).  Its mean duration is
     timed with HIP events on the batch's own stream for the u launch over
     every member of the largest grid size at level 0, after that batch has
     run (graph-replayed V-cycles cannot be bracketed per kernel, so the
     launch is repeated on its own).  Also reported: the FP32 sweep kernel
-    (k_rb32x<3,16,0>, three p' sweeps on the stored equation)."""
+    (k_rb32x<3,512,0,-1>, three p' sweeps on the stored equation)."""
     import mgfvm
     nmax = max(p.nfx for p in probs)
     big = [k for k in range(len(probs)) if probs[k].nfx == nmax]
@@ -125,12 +125,12 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
     nunk = (nmax - 1) * nmax * len(big_pairs)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
-            "kernel": f"k_rb32x<2,16,1>: fused u relaxation launch (FP64 assembly + 2 FP32 "
+            "kernel": f"k_rb32x<2,256,1,0>: fused u relaxation launch (FP64 assembly + 2 FP32 "
                       f"red-black sweeps) over the {len(big_pairs)} {nmax}^2 members, level 0",
             "mean_us": round(us, 2), "bytes_per_launch": nbytes,
             "bytes_per_unknown": round(nbytes / nunk, 2),
-            "sweep_kernel": {"kernel": "k_rb32x<3,16,0>: 3 FP32 red-black p' sweeps on the "
-                                       "stored equation, same members",
+            "sweep_kernel": {"kernel": "k_rb32x<3,512,0,-1>: 3 FP32 red-black p' sweeps on "
+                                       "the stored equation, same members",
                              "mean_us": round(us_s, 2), "bytes_per_launch": nbytes_s,
                              "effective_GBps": round(nbytes_s / (us_s * 1e-6) / 1e9, 1)}}
 

@@ -73,9 +73,10 @@ struct DevPhys {
     GQ double *cu, *cv, *cp, *csc;   /* coarse-grid corrections (ghosts stay 0) */    \
     GQ double *partial;              /* per-block partial sums of |div| */            \
     /* mixed precision (null in FP64 mode): FP32 coefficients, 1/diagonal,           \
-     * right-hand side (the FP64 residual for u, v, sc; -div for p'), and the two    \
-     * buffers of the FP32 correction sweeps */                                       \
-    GQ float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32;                  \
+     * right-hand side (the FP64 residual for u, v, sc; -div for p'), the two        \
+     * buffers of the FP32 correction sweeps, and the corrections of u and v,        \
+     * which stay pending (u + du, v + dv) until the update writes u and v */         \
+    GQ float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32, *du32, *dv32;    \
     /* obstacle data */                                                               \
     GQ const uint8_t *flags;                                                          \
     GQ const uint64_t *sccode;                                                        \
@@ -121,7 +122,8 @@ void mgf_launch_asm_p(const DevGroup &G, int record, hipStream_t st);
 void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
 /* field: 0 u, 1 v, 2 pp, 3 sc */
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
-void mgf_launch_update(const DevGroup &G, hipStream_t st);
+/* pending: mixed precision, u and v corrections still in du32 / dv32 */
+void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);
 void mgf_launch_restrict(const DevPair &R, hipStream_t st);

@@ -337,6 +337,10 @@ __global__ void k_pstash(const DevLevel *Ls)
     L.pstash[s] = in ? L.p[o] + P.relxp * (L.pp[o] - 0.0) : L.p[o];
 }
 
+/* `pending`: mixed precision, where the corrections of the u and v solves
+ * are still in du32 / dv32 over their swept ranges (the value of u there is
+ * u + du, formed exactly as the solve's own update would) */
+template <bool PENDING>
 __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
 {
     LEVEL(Ls);
@@ -345,8 +349,14 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
     const int nx = L.npx;
     const int ij = j * nx + i;
     const double pp = L.pp[ij];
-    L.u[ij] = L.u[ij] + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
-    L.v[ij] = L.v[ij] + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
+    double u = L.u[ij], v = L.v[ij];
+    if (PENDING) {
+        const float du = L.du32[ij], dv = L.dv32[ij];
+        u = i <= nx - 3 ? u + (double)du : u;
+        v = j <= L.npy - 3 ? v + (double)dv : v;
+    }
+    L.u[ij] = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
+    L.v[ij] = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
     if (L.flags[ij] & MGF_F_PCOPY)
         L.p[ij] = L.pstash[L.pslot[ij]];
     else
@@ -473,7 +483,10 @@ __device__ __forceinline__ double rs_sc(const DevLevelG &L, int i, int j, bool f
 #define TAU_UW (2 * BX + 1)
 #define TAU_W (2 * BX)
 #define TAU_H (2 * BY)
-__global__ void __launch_bounds__(BX * BY) k_tau(const DevLevel *Fs, const DevLevel *Cs)
+/* threads of a k_tau workgroup: more than the 64 x 4 coarse points, so each
+ * thread evaluates fewer fine residuals one after another */
+#define TAU_NT (2 * BX * BY)
+__global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
     __shared__ double U[TAU_H][TAU_UW];
@@ -495,22 +508,22 @@ __global__ void __launch_bounds__(BX * BY) k_tau(const DevLevel *Fs, const DevLe
     const int cu = icu >= ic0 ? 2 * (icu - ic0) + 2 : -1, ru = 2 * (jcc - jc0) + 1;
     const int cv = 2 * (icc - ic0) + 1, rv = jcv >= jc0 ? 2 * (jcv - jc0) + 2 : -1;
     const int cq = 2 * (icc - ic0) + 1, rq = 2 * (jcc - jc0) + 1;
-    for (int e = tid; e < TAU_H * TAU_UW; e += BX * BY) {
+    for (int e = tid; e < TAU_H * TAU_UW; e += TAU_NT) {
         const int r = e / TAU_UW, c = e - r * TAU_UW;
         U[r][c] = (c <= cu && r <= ru) ? rs_u(F, fi0 + c, fj0 + r, F.bodyx, true, m) : 0.0;
     }
-    for (int e = tid; e < (TAU_H + 1) * TAU_W; e += BX * BY) {
+    for (int e = tid; e < (TAU_H + 1) * TAU_W; e += TAU_NT) {
         const int r = e / TAU_W, c = e - r * TAU_W;
         V[r][c] = (c <= cv && r <= rv) ? rs_v(F, fi0 + c, fj0 + r, F.bodyy, true, m) : 0.0;
     }
-    for (int e = tid; e < TAU_H * TAU_W; e += BX * BY) {
+    for (int e = tid; e < TAU_H * TAU_W; e += TAU_NT) {
         const int r = e / TAU_W, c = e - r * TAU_W;
         Q[r][c] = (c <= cq && r <= rq) ? rs_sc(F, fi0 + c, fj0 + r, true, m) : 0.0;
     }
     __syncthreads();
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int ic = ic0 + tx, jc = jc0 + ty;
-    if (ic > nxc - 2 || jc > nyc - 2) return;
+    if (ty >= BY || ic > nxc - 2 || jc > nyc - 2) return;
     const int ijc = jc * nxc + ic;
     const int c1 = 2 * tx + 1, r1 = 2 * ty + 1;     /* fine (2ic, 2jc) in the tables */
     if (ic <= nxc - 3) {
@@ -780,13 +793,18 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
                            G.d, field);
 }
 
-void mgf_launch_update(const DevGroup &G, hipStream_t st)
+void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st)
 {
     if (G.max_npcopy > 0)
         hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
-    hipLaunchKernelGGL(k_update, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    if (pending)
+        hipLaunchKernelGGL(k_update<true>, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY),
+                           0, st, G.d);
+    else
+        hipLaunchKernelGGL(k_update<false>, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY),
+                           0, st, G.d);
 }
 
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st)
@@ -809,7 +827,7 @@ void mgf_launch_restrict(const DevPair &R, hipStream_t st)
 {
     dim3 g = tile_grid(R.npxc - 2, R.npyc - 2, R.count);
     hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, R.f, R.c);
-    hipLaunchKernelGGL(k_tau, g, dim3(BX, BY), 0, st, R.f, R.c);
+    hipLaunchKernelGGL(k_tau, g, dim3(BX, TAU_NT / BX), 0, st, R.f, R.c);
 }
 
 void mgf_launch_prolong(const DevPair &R, hipStream_t st)

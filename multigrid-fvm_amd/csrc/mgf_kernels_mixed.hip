@@ -88,7 +88,16 @@ __device__ __forceinline__ Pt32 asm_pt_p(const DevLevelG &L, int i, int j, doubl
     const DevPhys &P = L.P;
     const int nx = L.npx;
     const int ij = j * nx + i, ijs = ij - nx;
-    double div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
+    /* u and v carry their pending corrections over their swept ranges (u
+     * faces 1..npx-3, v faces 1..npy-3): the values are u + du, v + dv,
+     * formed as the solves' own update would form them */
+    const double u0 = L.u[ij], u1 = L.u[ij - 1], v0 = L.v[ij], v1 = L.v[ijs];
+    const float du0 = L.du32[ij], du1 = L.du32[ij - 1], dv0 = L.dv32[ij], dv1 = L.dv32[ijs];
+    const double ue = i <= nx - 3 ? u0 + (double)du0 : u0;
+    const double uw = i - 1 >= 1 ? u1 + (double)du1 : u1;
+    const double vn = j <= L.npy - 3 ? v0 + (double)dv0 : v0;
+    const double vs = j - 1 >= 1 ? v1 + (double)dv1 : v1;
+    double div = ((L.ax * ue - L.ax * uw) + L.ay * vn) - L.ay * vs;
     double su = -div;
     mag = fabs(div);
     double aw = L.axax / L.apu[ij - 1];
@@ -186,6 +195,9 @@ __global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, 
     extents(L, field, nx, ny);
     const gflt *src = launch == 0 ? nullptr : ((launch & 1) ? L.db32 : L.da32);
     gflt *dst = (launch & 1) ? L.da32 : L.db32;
+    /* the last launch of u / v leaves the correction pending in du32 / dv32 */
+    if (last && field == 0) dst = L.du32;
+    if (last && field == 1) dst = L.dv32;
     const bool pfield = field == 2;
     const int npx = L.npx;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * TY - H;
@@ -310,8 +322,9 @@ __global__ void __launch_bounds__(BX * BY) k_jac32(const DevLevel *Ls, int field
     dst[ij] = (s + L.r32[ij]) * L.iap32[ij];
 }
 
-/* phi += d (u, v, sc) or pp = d, in FP64, over the swept range; nl is the
- * number of sweep launches (its parity names the buffer holding d) */
+/* the result of nl sweep launches (its parity names the buffer holding d)
+ * over the swept range: sc += d in FP64; pp = d in FP64; for u and v the
+ * correction is left pending in du32 / dv32 (zero without sweeps) */
 __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int field, int nl)
 {
     LEVEL(Ls);
@@ -320,7 +333,11 @@ __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int fi
     RANGE2D(1, nx - 1, 1, ny - 1);
     const gflt *d = (nl & 1) ? L.db32 : L.da32;
     const int ij = j * L.npx + i;
-    if (field == 2)
+    if (field == 0)
+        L.du32[ij] = nl > 0 ? d[ij] : 0.f;
+    else if (field == 1)
+        L.dv32[ij] = nl > 0 ? d[ij] : 0.f;
+    else if (field == 2)
         phi[ij] = nl > 0 ? (double)d[ij] : 0.0;
     else
         phi[ij] = phi[ij] + (double)d[ij];
@@ -481,9 +498,11 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
             launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
         done += n;
     }
+    /* p': the last launch wrote pp; u, v: the last launch left the
+     * correction pending in du32 / dv32; sc: apply it */
     if (field == 2)
         mgf_launch_reduce_err(G, record, st);
-    else
+    else if (field == 3)
         hipLaunchKernelGGL(k_apply_mx, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, field, launch);
 }
@@ -523,8 +542,9 @@ void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, 
     sweep_extents(G, field, nx, ny);
     if (nx - 1 < 1 || ny - 1 < 1 || (isolve != 1 && isolve != 2)) return;
     const int nl = mgf_launch_sweeps_mixed(G, field, isolve, nsw, cfg, st);
-    /* without sweeps the correction is zero: u, v, sc stay, pp becomes 0 */
-    if (nl > 0 || field == 2)
+    /* without sweeps the correction is zero: sc stays, pp becomes 0, du / dv
+     * become 0 */
+    if (nl > 0 || field != 3)
         hipLaunchKernelGGL(k_apply_mx, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, field, nl);
 }

@@ -113,7 +113,7 @@ static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
     size_t t = 27 * align256(n * sizeof(double));  /* 23 arrays + 4 initial-state copies */
-    if (mixed) t += 8 * align256(n * sizeof(float));
+    if (mixed) t += 10 * align256(n * sizeof(float));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
     t += align256(n * sizeof(uint64_t));            /* sccode */
@@ -173,9 +173,9 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl)
         L.partial = (double *)take((size_t)L.nblk * sizeof(double));
         L.isolve = c->isolve;
         if (s->opt.precision == MGF_PREC_MIXED) {
-            float **a32[8] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
-                              &L.da32, &L.db32};
-            for (int k = 0; k < 8; ++k) *a32[k] = (float *)take(n * sizeof(float));
+            float **a32[10] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
+                               &L.da32, &L.db32, &L.du32, &L.dv32};
+            for (int k = 0; k < 10; ++k) *a32[k] = (float *)take(n * sizeof(float));
         }
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
@@ -532,7 +532,7 @@ static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg)
     mgf_launch_relax_field_mixed(G, 0, s->isolve, s->nswpm, cfg, 0, st);
     mgf_launch_relax_field_mixed(G, 1, s->isolve, s->nswpm, cfg, 0, st);
     mgf_launch_relax_field_mixed(G, 2, s->isolve, s->nswpp, cfg, leg == 0, st);
-    mgf_launch_update(G, st);
+    mgf_launch_update(G, 1, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) {
         mgf_launch_outflow(G, st);
@@ -557,7 +557,7 @@ static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg)
     mgf_launch_smooth(G, 1, s->isolve, s->nswpm, st);
     mgf_launch_asm_p(G, leg == 0, st);
     mgf_launch_smooth(G, 2, s->isolve, s->nswpp, st);
-    mgf_launch_update(G, st);
+    mgf_launch_update(G, 0, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) {
         mgf_launch_outflow(G, st);
