@@ -48,9 +48,11 @@ sys.path.insert(0, os.path.join(ROOT, "multigrid-fvm_amd"))
 METRIC = ("converged solves/s at FP64 residual 1e-6, batched-mixed vs plain GPU; "
           "% HBM peak")
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_rb32 launch over the 512^2 members, from rocprofv3 FETCH_SIZE
-# and WRITE_SIZE passes scaled by a 1 GiB device copy (bench.py --roofline-only)
-TRAFFIC = None
+# HBM bytes per launch of the roofline kernel (k_rb32x<2,256,1,0> over the 84
+# 512^2 members): rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
+# `bench.py --roofline-only --reps 10`, each scaled by what the counter reads
+# for a 1 GiB device copy (profiles/traffic_relax_u_512x84.md)
+TRAFFIC = 1425.6e6
 WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
             "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
             "each obstacle moved by a seeded quarter/half/full cell), Re in "
