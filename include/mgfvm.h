@@ -114,6 +114,37 @@ void mgf_options_default(mgf_options *o);
 const char *mgf_error_string(void);
 int mgf_abi_version(void);
 
+/* ---- seeded obstacle patterns (host only) ------------------------------ */
+
+/* Each returns the number of rectangles of the pattern and writes the first
+ * `cap` of them to rects (4 ints each: the 1-based `if il jf jl` of a
+ * pattern.txt line, on the obstacle base level), or a negative status.
+ * Rectangles are one per blocked cell (or particle), in the order the
+ * reference's generators write them; the first index is the solver's i (x).
+ * free_edge_columns: keep the first and last i column free, so a plug inlet
+ * and outlet keep matching fluxes.  The reference seeds rand() with the
+ * time; these draw from splitmix64-seeded xoshiro256**. */
+
+/* Every base cell blocked with probability 1 - porosity (one draw per cell,
+ * i-major).  No counterpart in the reference; the random configurations of
+ * the benchmark are defined this way. */
+int mgf_pattern_bernoulli(int n_base, double porosity, uint64_t seed, int free_edge_columns,
+                          int *rects, int cap);
+/* create_patterns_staggered.c:71-188: 1x1 base obstacles with spacing_x /
+ * spacing_y free cells between them on a grid refined by `scale` (4, 2, 1:
+ * quarter, half, full cell shifts); variant 1 the inline array (its
+ * pattern_1), 2 the staggered array (pattern_2), >= 3 the staggered array
+ * with every obstacle staying or moving one sub-cell unit (pattern_3...).
+ * The rectangles live on the (n_base * scale)^2 grid. */
+int mgf_pattern_staggered(int n_base, int spacing_x, int spacing_y, int scale, int variant,
+                          uint64_t seed, int free_edge_columns, int *rects, int cap);
+/* create_patterns_random.c:58-138: 3x3, 2x2 and 1x1 particles (pct3, pct2,
+ * pct1 percent of the count that fills `ratio` of the n_grid^2 cells),
+ * largest first, each placed at a random free spot (up to 10000 attempts,
+ * dropped when none fits); one rectangle per particle. */
+int mgf_pattern_particles(int n_grid, double ratio, double pct1, double pct2, double pct3,
+                          uint64_t seed, int free_edge_columns, int *rects, int cap);
+
 /* ---- solver lifetime -------------------------------------------------- */
 
 /* Build the level hierarchy, initial/boundary state and coarse geometry on

@@ -102,17 +102,7 @@ int main(int argc, char **argv)
     double *pr = (double *)malloc(n * sizeof(double)), *sc = (double *)malloc(n * sizeof(double));
     unsigned char *obs = (unsigned char *)calloc(n, 1);
     mgf_get_fields(s, 0, u, v, pr, NULL, sc);
-    if (p->nobs > 0 && p->ig_obs >= 1) {
-        int fac = 1 << (p->ig_obs - 1);
-        for (int k = 0; k < p->nobs; ++k) {
-            const int *ob = p->obs + 4 * k;
-            int is = (ob[0] + p->ibeg_obs - 1) * fac + 1, ie = (ob[1] + p->ibeg_obs) * fac;
-            int js = (ob[2] - 1) * fac + 1, je = ob[3] * fac;
-            for (int j = js; j <= je; ++j)
-                for (int i = is; i <= ie; ++i)
-                    if (i >= 0 && i < npx && j >= 0 && j < npy) obs[(size_t)j * npx + i] = 1;
-        }
-    }
+    mgf_obstacle_mask(p, npx, npy, obs);
     /* The reference writes its outputs when convergence_data[icyc] < tol
      * after the loop, which also holds when the cycle limit was reached
      * (the entry one past the last cycle is still 0.0, V6:265); it writes

@@ -12,8 +12,25 @@
  */
 #include <math.h>
 #include <stdio.h>
+#include <string.h>
 
 #include "mgf_output.h"
+#include "mgfvm.h"
+
+void mgf_obstacle_mask(const mgf_problem *p, int npx, int npy, unsigned char *obs)
+{
+    memset(obs, 0, (size_t)npx * npy);
+    if (p->nobs <= 0 || p->ig_obs < 1) return;
+    const int fac = 1 << (p->ig_obs - 1);
+    for (int k = 0; k < p->nobs; ++k) {
+        const int *ob = p->obs + 4 * k;
+        const int is = (ob[0] + p->ibeg_obs - 1) * fac + 1, ie = (ob[1] + p->ibeg_obs) * fac;
+        const int js = (ob[2] - 1) * fac + 1, je = ob[3] * fac;
+        for (int j = js; j <= je; ++j)
+            for (int i = is; i <= ie; ++i)
+                if (i >= 0 && i < npx && j >= 0 && j < npy) obs[(size_t)j * npx + i] = 1;
+    }
+}
 
 int mgf_write_vtk(const char *fname, int npx, int npy, double dx, double dy, const double *u,
                   const double *v, const double *p, const double *sc, const unsigned char *obs,

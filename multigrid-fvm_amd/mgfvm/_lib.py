@@ -23,7 +23,8 @@ EXPORTS = [
     "mgf_vcycle", "mgf_solve", "mgf_get_field", "mgf_set_field", "mgf_get_fields",
     "mgf_time_kernel", "mgf_reset", "mgf_synchronize", "mgf_batch_create", "mgf_batch_size",
     "mgf_batch_num_levels", "mgf_batch_level_dims", "mgf_batch_solve", "mgf_batch_history",
-    "mgf_batch_get_field",
+    "mgf_batch_get_field", "mgf_pattern_bernoulli", "mgf_pattern_staggered",
+    "mgf_pattern_particles",
 ]
 
 PREC_FP64 = 0
@@ -110,6 +111,10 @@ def lib():
         L.mgf_batch_solve.argtypes = [vp, P_i, P_i]
         L.mgf_batch_history.argtypes = [vp, ip, P_d, ip]
         L.mgf_batch_get_field.argtypes = [vp, ip, ip, cp, P_d]
+        u64 = ctypes.c_uint64
+        L.mgf_pattern_bernoulli.argtypes = [ip, dp, u64, ip, P_i, ip]
+        L.mgf_pattern_staggered.argtypes = [ip, ip, ip, ip, ip, u64, ip, P_i, ip]
+        L.mgf_pattern_particles.argtypes = [ip, dp, dp, dp, dp, u64, ip, P_i, ip]
         _lib = L
     return _lib
 
@@ -333,3 +338,31 @@ class Batch:
         self._check(lib().mgf_time_kernel(self.h, kernel, ig, reps, ctypes.byref(us),
                                           ctypes.byref(b)))
         return us.value, b.value
+
+
+def _rects(fn, *args):
+    n = fn(*args, None, 0)
+    if n < 0:
+        raise MgfError(n, lib().mgf_error_string().decode())
+    buf = (ctypes.c_int * (4 * max(n, 1)))()
+    fn(*args, buf, n)
+    return [tuple(buf[4 * k:4 * k + 4]) for k in range(n)]
+
+
+def pattern_bernoulli(n_base: int, porosity: float, seed: int, free_edge_columns: bool = True):
+    """Rectangles of mgf_pattern_bernoulli (the native generator)."""
+    return _rects(lib().mgf_pattern_bernoulli, n_base, porosity, seed, int(free_edge_columns))
+
+
+def pattern_staggered(n_base: int, spacing_x: int, spacing_y: int, scale: int, variant: int,
+                      seed: int, free_edge_columns: bool = True):
+    """Rectangles of mgf_pattern_staggered (the native generator)."""
+    return _rects(lib().mgf_pattern_staggered, n_base, spacing_x, spacing_y, scale, variant, seed,
+                  int(free_edge_columns))
+
+
+def pattern_particles(n_grid: int, ratio: float, pct1: float, pct2: float, pct3: float,
+                      seed: int, free_edge_columns: bool = True):
+    """Rectangles of mgf_pattern_particles (the native generator)."""
+    return _rects(lib().mgf_pattern_particles, n_grid, ratio, pct1, pct2, pct3, seed,
+                  int(free_edge_columns))
