@@ -19,6 +19,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "mgf_dual.h"
+
 #define MGF_MAX_LEVELS 24
 
 /* device-resident control block of one instance's solve */
@@ -45,7 +47,14 @@ struct DevPhys {
  * flat loads and stores, which also count against the LDS wait counter: a
  * kernel that mixes them with LDS traffic waits for its global loads at
  * every LDS wait. */
+#ifdef MGF_TANGENT
+/* dual numbers are structs, and clang does not copy a struct out of
+ * global-address-space storage into a generic object: generic pointers in
+ * the sensitivity build */
+#define MGF_G
+#else
 #define MGF_G __attribute__((address_space(1)))
+#endif
 #define MGF_LEVEL_BODY(GQ)                                                              \
     int npx, npy;                                                                     \
     int ig;               /* level index within its instance (0 = finest) */          \
@@ -55,22 +64,22 @@ struct DevPhys {
     int nblk;             /* workgroups of the residual kernel (partials) */          \
     int isolve;           /* 1 Jacobi, 2 red-black */                                 \
     /* metrics, each computed on the host with the reference's expression */         \
-    double ax, ay;        /* areax (= dy), areay (= dx) */                            \
-    double du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, walls */   \
-    double ds_x, ds_y;    /* amusc*(areax/dx), amusc*(areay/dy): coefsc, its adds */  \
-    double axax, ayay;    /* areax*areax, areay*areay: coefp */                       \
-    double bodyx, bodyy;  /* (dpdxm*areax)*areay, (dpdym*areax)*areay */              \
+    real_t ax, ay;        /* areax (= dy), areay (= dx) */                            \
+    real_t du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, walls */   \
+    real_t ds_x, ds_y;    /* amusc*(areax/dx), amusc*(areay/dy): coefsc, its adds */  \
+    real_t axax, ayay;    /* areax*areax, areay*areay: coefp */                       \
+    real_t bodyx, bodyy;  /* (dpdxm*areax)*areay, (dpdym*areax)*areay */              \
     DevPhys P;            /* the instance's physics / boundary types */               \
     GQ DevCtl *ctl;       /* the instance's control block */                          \
     /* state */                                                                       \
-    GQ double *u, *v, *p, *pp, *sc;                                                   \
-    GQ double *resu, *resv, *ressc;  /* FAS tau terms (zero on level 0) */            \
-    GQ double *apu, *apv;            /* momentum diagonals (persist; ghosts 1e30) */  \
+    GQ real_t *u, *v, *p, *pp, *sc;                                                   \
+    GQ real_t *resu, *resv, *ressc;  /* FAS tau terms (zero on level 0) */            \
+    GQ real_t *apu, *apv;            /* momentum diagonals (persist; ghosts 1e30) */  \
     /* scratch shared by the equations, as in the reference */                        \
-    GQ double *cx, *cy;                                                               \
-    GQ double *aw, *ae, *as, *an, *ap, *su;                                           \
-    GQ double *tmp;                  /* smoother ping-pong */                         \
-    GQ double *cu, *cv, *cp, *csc;   /* coarse-grid corrections (ghosts stay 0) */    \
+    GQ real_t *cx, *cy;                                                               \
+    GQ real_t *aw, *ae, *as, *an, *ap, *su;                                           \
+    GQ real_t *tmp;                  /* smoother ping-pong */                         \
+    GQ real_t *cu, *cv, *cp, *csc;   /* coarse-grid corrections (ghosts stay 0) */    \
     GQ double *partial;              /* per-block partial sums of |div| */            \
     /* mixed precision (null in FP64 mode): FP32 coefficients, 1/diagonal,           \
      * right-hand side (the FP64 residual for u, v, sc; -div for p'), the two        \
@@ -82,7 +91,7 @@ struct DevPhys {
     GQ const uint64_t *sccode;                                                        \
     GQ const int *pslot;                                                              \
     GQ const int *porig;                                                              \
-    GQ double *pstash;
+    GQ real_t *pstash;
 
 struct DevLevel {
     MGF_LEVEL_BODY()
@@ -93,7 +102,8 @@ struct DevLevelG {
 static_assert(sizeof(DevLevel) == sizeof(DevLevelG), "descriptor views differ in layout");
 static_assert(offsetof(DevLevel, pstash) == offsetof(DevLevelG, pstash),
               "descriptor views differ in layout");
-typedef MGF_G double gdbl;
+/* the FP64 path's scalar in the global address space */
+typedef MGF_G real_t gdbl;
 typedef MGF_G float gflt;
 
 /* a set of same-shape levels launched together */
@@ -113,6 +123,18 @@ struct DevPair {
     int count;
     int npxf, npyf, npxc, npyc;
 };
+
+/* solver internals shared with the sensitivity calls (mgf_solver.hip) */
+struct mgf_seed;
+struct mgf_problem;
+struct mgf_options;
+struct mgf_solver;
+int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_options *opt,
+                            mgf_solver **out, const mgf_seed *seeds);
+int mgf_field_array(mgf_solver *s, int b, int ig, const char *name, real_t **ptr, size_t *n,
+                    hipStream_t *st);
+const mgf_problem *mgf_instance_problem(mgf_solver *s, int b);
+int mgf_set_error(mgf_solver *s, int code, const char *msg);
 
 /* launch wrappers (mgf_kernels.hip) */
 void mgf_launch_fluxes(const DevGroup &G, hipStream_t st);
@@ -160,7 +182,7 @@ void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, 
  * the number of launches */
 int mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
                             hipStream_t st);
-void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st);
+void mgf_launch_fill(real_t *a, double v, size_t n, hipStream_t st);
 int mgf_partial_blocks(int npx, int npy);
 
 #endif

@@ -44,8 +44,8 @@ __global__ void __launch_bounds__(BX * BY) k_asm_u(const DevLevel *Ls)
     RANGE2D(1, L.npx - 3, 1, L.npy - 2);
     const int ij = j * L.npx + i;
     Coef c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
-    double su = L.ax * (L.p[ij] - L.p[ij + 1]);
-    double ap = c.ap / P.relxm;
+    real_t su = L.ax * (L.p[ij] - L.p[ij + 1]);
+    real_t ap = c.ap / P.relxm;
     su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
     ap = ap + ((L.flags[ij] & MGF_F_SPU) ? 1.0e30 : 0.0);
     L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
@@ -62,8 +62,8 @@ __global__ void __launch_bounds__(BX * BY) k_asm_v(const DevLevel *Ls)
     RANGE2D(1, L.npx - 2, 1, L.npy - 3);
     const int ij = j * L.npx + i;
     Coef c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
-    double su = L.ay * (L.p[ij] - L.p[ij + L.npx]);
-    double ap = c.ap / P.relxm;
+    real_t su = L.ay * (L.p[ij] - L.p[ij + L.npx]);
+    real_t ap = c.ap / P.relxm;
     su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
     ap = ap + ((L.flags[ij] & MGF_F_SPV) ? 1.0e30 : 0.0);
     L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
@@ -102,14 +102,14 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
     double mag = 0.0;
     if (i <= nx - 2 && j <= L.npy - 2) {
         const int ij = j * nx + i, ijs = ij - nx;
-        double div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
-        double su = -div;
-        mag = fabs(div);
-        double aw = L.axax / L.apu[ij - 1];
-        double ae = L.axax / L.apu[ij];
-        double as = L.ayay / L.apv[ijs];
-        double an = L.ayay / L.apv[ij];
-        double ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
+        real_t div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
+        real_t su = -div;
+        mag = val(fabs(div));
+        real_t aw = L.axax / L.apu[ij - 1];
+        real_t ae = L.axax / L.apu[ij];
+        real_t as = L.ayay / L.apv[ijs];
+        real_t an = L.ayay / L.apv[ij];
+        real_t ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
         const double diag = 1.0e30;
         if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
         if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
@@ -161,12 +161,12 @@ __global__ void __launch_bounds__(BX * BY) k_jacobi(const DevLevel *Ls, int fiel
     RANGE2D(1, nx - 1, 1, ny - 1);
     const int npx = L.npx;
     const int ij = j * npx + i;
-    const double W = (i - 1 >= 1) ? src[ij - 1] : base[ij - 1];
-    const double E = (i + 1 <= nx - 1) ? src[ij + 1] : base[ij + 1];
-    const double S = (j - 1 >= 1) ? src[ij - npx] : base[ij - npx];
-    const double N = (j + 1 <= ny - 1) ? src[ij + npx] : base[ij + npx];
-    const double sum_nb = ((L.aw[ij] * W + L.ae[ij] * E) + L.as[ij] * S) + L.an[ij] * N;
-    const double g = (sum_nb + L.su[ij]) / (L.ap[ij] + 1.0e-30);
+    const real_t W = (i - 1 >= 1) ? src[ij - 1] : base[ij - 1];
+    const real_t E = (i + 1 <= nx - 1) ? src[ij + 1] : base[ij + 1];
+    const real_t S = (j - 1 >= 1) ? src[ij - npx] : base[ij - npx];
+    const real_t N = (j + 1 <= ny - 1) ? src[ij + npx] : base[ij + npx];
+    const real_t sum_nb = ((L.aw[ij] * W + L.ae[ij] * E) + L.as[ij] * S) + L.an[ij] * N;
+    const real_t g = (sum_nb + L.su[ij]) / (L.ap[ij] + 1.0e-30);
     const double wj = 1.0;
     dst[ij] = (1.0 - wj) * src[ij] + wj * g;
 }
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(BX * BY) k_copy_range(const DevLevel *Ls, int 
 #define RB_RING (2 * (BX + 2) + 2 * RB_TY)
 
 struct Pt {
-    double aw, ae, as, an, ap, su;
+    real_t aw, ae, as, an, ap, su;
 };
 
 __device__ __forceinline__ void rb_ring_pos(int k, int &r, int &c)
@@ -217,18 +217,18 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
     const gdbl *base = field_of(L, field, nx, ny);
     const gdbl *src = parity ? L.tmp : base;
     gdbl *dst = parity ? (gdbl *)base : L.tmp;
-    __shared__ double t[RB_TY + 4][BX + 4];
+    __shared__ real_t t[RB_TY + 4][BX + 4];
     const int npx = L.npx, npy = L.npy;
     const int i0 = 1 + T.x * BX, j0 = 1 + T.y * RB_TY;
     const int tid = threadIdx.y * BX + threadIdx.x;
     constexpr int NST = ((RB_TY + 4) * (BX + 4) + BX * BY - 1) / (BX * BY);
 
     /* 1. issue every global load: halo staging, own points, one ring point */
-    double sv[NST];
+    real_t sv[NST];
 #pragma unroll
     for (int q = 0; q < NST; ++q) {
         const int k = tid + q * BX * BY;
-        double val = 0.0;
+        real_t val = 0.0;
         if (k < (RB_TY + 4) * (BX + 4)) {
             const int r = k / (BX + 4), c = k % (BX + 4);
             const int gi = i0 - 2 + c, gj = j0 - 2 + r;
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
     }
     __syncthreads();
     /* 3. red points: own and ring */
-    double res[RB_Q];
+    real_t res[RB_Q];
 #pragma unroll
     for (int q = 0; q < RB_Q; ++q) {
         const int r = 2 + threadIdx.y + BY * q, c = 2 + threadIdx.x;
@@ -282,14 +282,14 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
         res[q] = t[r][c];
         if (live[q] && ((gi + gj) & 1) == 0) {
             const Pt &o = own[q];
-            const double sum_nb = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
+            const real_t sum_nb = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
                                   o.an * t[r + 1][c];
             res[q] = (sum_nb + o.su) / o.ap;
         }
     }
-    double rv = 0.0;
+    real_t rv = 0.0;
     if (ring) {
-        const double sum_nb = ((rp.aw * t[rr][rc - 1] + rp.ae * t[rr][rc + 1]) +
+        const real_t sum_nb = ((rp.aw * t[rr][rc - 1] + rp.ae * t[rr][rc + 1]) +
                                rp.as * t[rr - 1][rc]) + rp.an * t[rr + 1][rc];
         rv = (sum_nb + rp.su) / rp.ap;
     }
@@ -308,10 +308,10 @@ __global__ void __launch_bounds__(BX * BY) k_rb_sweep(const DevLevel *Ls, int fi
         const int r = 2 + threadIdx.y + BY * q, c = 2 + threadIdx.x;
         const int gj = j0 + threadIdx.y + BY * q;
         if (!live[q]) continue;
-        double val = res[q];
+        real_t val = res[q];
         if (((gi + gj) & 1) == 1) {
             const Pt &o = own[q];
-            const double sum_nb = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
+            const real_t sum_nb = ((o.aw * t[r][c - 1] + o.ae * t[r][c + 1]) + o.as * t[r - 1][c]) +
                                   o.an * t[r + 1][c];
             val = (sum_nb + o.su) / o.ap;
         }
@@ -348,8 +348,8 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
     RANGE2D(1, L.npx - 2, 1, L.npy - 2);
     const int nx = L.npx;
     const int ij = j * nx + i;
-    const double pp = L.pp[ij];
-    double u = L.u[ij], v = L.v[ij];
+    const real_t pp = L.pp[ij];
+    real_t u = L.u[ij], v = L.v[ij];
     if (PENDING) {
         const float du = L.du32[ij], dv = L.dv32[ij];
         u = i <= nx - 3 ? u + (double)du : u;
@@ -431,13 +431,13 @@ __global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const Dev
 
 /* momentum residual at a u face with un-relaxed, un-penalised coefficients
  * (restru, V6:1496-1510 fine with resu, 1532-1545 coarse without) */
-__device__ __forceinline__ double rs_u(const DevLevelG &L, int i, int j, double body, bool fine,
+__device__ __forceinline__ real_t rs_u(const DevLevelG &L, int i, int j, real_t body, bool fine,
                                        bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefu_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
     const gdbl *u = L.u;
-    double r = ((((c.aw * u[ij - 1] + c.ae * u[ij + 1]) + c.as * u[ij - nx]) + c.an * u[ij + nx]) -
+    real_t r = ((((c.aw * u[ij - 1] + c.ae * u[ij + 1]) + c.as * u[ij - nx]) + c.an * u[ij + nx]) -
                 c.ap * u[ij]) + L.ax * (L.p[ij] - L.p[ij + 1]);
     if (fine) r = r + L.resu[ij];
     r = r + body;
@@ -446,13 +446,13 @@ __device__ __forceinline__ double rs_u(const DevLevelG &L, int i, int j, double 
     return r;
 }
 
-__device__ __forceinline__ double rs_v(const DevLevelG &L, int i, int j, double body, bool fine,
+__device__ __forceinline__ real_t rs_v(const DevLevelG &L, int i, int j, real_t body, bool fine,
                                        bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefv_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
     const gdbl *v = L.v;
-    double r = ((((c.aw * v[ij - 1] + c.ae * v[ij + 1]) + c.as * v[ij - nx]) + c.an * v[ij + nx]) -
+    real_t r = ((((c.aw * v[ij - 1] + c.ae * v[ij + 1]) + c.as * v[ij - nx]) + c.an * v[ij + nx]) -
                 c.ap * v[ij]) + L.ay * (L.p[ij] - L.p[ij + nx]);
     if (fine) r = r + L.resv[ij];
     r = r + body;
@@ -461,13 +461,13 @@ __device__ __forceinline__ double rs_v(const DevLevelG &L, int i, int j, double 
     return r;
 }
 
-__device__ __forceinline__ double rs_sc(const DevLevelG &L, int i, int j, bool fine,
+__device__ __forceinline__ real_t rs_sc(const DevLevelG &L, int i, int j, bool fine,
                                         bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
     Coef c = coefsc_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j, L.flags[ij]);
     const gdbl *s = L.sc;
-    double r = (((c.aw * s[ij - 1] + c.ae * s[ij + 1]) + c.as * s[ij - nx]) + c.an * s[ij + nx]) -
+    real_t r = (((c.aw * s[ij - 1] + c.ae * s[ij + 1]) + c.as * s[ij - nx]) + c.an * s[ij + nx]) -
                c.ap * s[ij];
     if (fine) r = r + L.ressc[ij];
     if (mask && (L.flags[ij] & MGF_F_SPSC)) r = 0.0;
@@ -489,9 +489,9 @@ __device__ __forceinline__ double rs_sc(const DevLevelG &L, int i, int j, bool f
 __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
-    __shared__ double U[TAU_H][TAU_UW];
-    __shared__ double V[TAU_H + 1][TAU_W];
-    __shared__ double Q[TAU_H][TAU_W];
+    __shared__ real_t U[TAU_H][TAU_UW];
+    __shared__ real_t V[TAU_H + 1][TAU_W];
+    __shared__ real_t Q[TAU_H][TAU_W];
     const int nxc = C.npx, nyc = C.npy;
     const int ic0 = 1 + T.x * BX, jc0 = 1 + T.y * BY;
     if (ic0 > nxc - 2 || jc0 > nyc - 2) return;          /* padding tiles */
@@ -527,23 +527,23 @@ __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLev
     const int ijc = jc * nxc + ic;
     const int c1 = 2 * tx + 1, r1 = 2 * ty + 1;     /* fine (2ic, 2jc) in the tables */
     if (ic <= nxc - 3) {
-        double a = U[r1][c1], b = U[r1 - 1][c1];
-        double w0 = U[r1][c1 - 1], w1 = U[r1 - 1][c1 - 1];
-        double e0 = U[r1][c1 + 1], e1 = U[r1 - 1][c1 + 1];
-        double rc = rs_u(C, ic, jc, C.bodyx, false);
+        real_t a = U[r1][c1], b = U[r1 - 1][c1];
+        real_t w0 = U[r1][c1 - 1], w1 = U[r1 - 1][c1 - 1];
+        real_t e0 = U[r1][c1 + 1], e1 = U[r1 - 1][c1 + 1];
+        real_t rc = rs_u(C, ic, jc, C.bodyx, false);
         C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rc;
     }
     if (jc <= nyc - 3) {
-        double a = V[r1][c1], b = V[r1][c1 - 1];
-        double s0 = V[r1 - 1][c1], s1 = V[r1 - 1][c1 - 1];
-        double n0 = V[r1 + 1][c1], n1 = V[r1 + 1][c1 - 1];
-        double rc = rs_v(C, ic, jc, C.bodyy, false);
+        real_t a = V[r1][c1], b = V[r1][c1 - 1];
+        real_t s0 = V[r1 - 1][c1], s1 = V[r1 - 1][c1 - 1];
+        real_t n0 = V[r1 + 1][c1], n1 = V[r1 + 1][c1 - 1];
+        real_t rc = rs_v(C, ic, jc, C.bodyy, false);
         C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rc;
     }
     {
-        double a = Q[r1][c1], b = Q[r1 - 1][c1];
-        double c2 = Q[r1][c1 - 1], d = Q[r1 - 1][c1 - 1];
-        double rc = rs_sc(C, ic, jc, false);
+        real_t a = Q[r1][c1], b = Q[r1 - 1][c1];
+        real_t c2 = Q[r1][c1 - 1], d = Q[r1 - 1][c1 - 1];
+        real_t rc = rs_sc(C, ic, jc, false);
         C.ressc[ijc] = (((a + b) + c2) + d) - rc;
     }
 }
@@ -569,25 +569,25 @@ __global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const De
      * level carry no information about the finer level and are dropped */
     const uint8_t cfl = C.ext ? C.flags[ijc] : (uint8_t)0;
     if (ic <= nxc - 3) {
-        double Ru = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0E-30);
-        double cu = C.u[ijc] - Ru;
+        real_t Ru = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0E-30);
+        real_t cu = C.u[ijc] - Ru;
         if (cfl & MGF_F_SPU) cu = 0.0;
         C.cu[ijc] = cu;
     }
     if (jc <= nyc - 3) {
-        double Rv = (F.v[ij] * F.ay + F.v[ij - 1] * F.ay) / (C.ay + 1.0E-30);
-        double cv = C.v[ijc] - Rv;
+        real_t Rv = (F.v[ij] * F.ay + F.v[ij - 1] * F.ay) / (C.ay + 1.0E-30);
+        real_t cv = C.v[ijc] - Rv;
         if (cfl & MGF_F_SPV) cv = 0.0;
         C.cv[ijc] = cv;
     }
     {
-        double Rp = 0.25 * (((F.p[ij] + F.p[ij - 1]) + F.p[ijs]) + F.p[ijs - 1]);
-        double cp = C.p[ijc] - Rp;
+        real_t Rp = 0.25 * (((F.p[ij] + F.p[ij - 1]) + F.p[ijs]) + F.p[ijs - 1]);
+        real_t cp = C.p[ijc] - Rp;
         if (F.flags[ij] & MGF_F_SPU) cp = 0.0;
         if (cfl & MGF_F_SPSC) cp = 0.0;
         C.cp[ijc] = cp;
-        double Rs = 0.25 * (((F.sc[ij] + F.sc[ij - 1]) + F.sc[ijs]) + F.sc[ijs - 1]);
-        double cs = C.sc[ijc] - Rs;
+        real_t Rs = 0.25 * (((F.sc[ij] + F.sc[ij - 1]) + F.sc[ijs]) + F.sc[ijs - 1]);
+        real_t cs = C.sc[ijc] - Rs;
         if (cfl & MGF_F_SPSC) cs = 0.0;
         C.csc[ijc] = cs;
     }
@@ -596,24 +596,24 @@ __global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const De
 /* face correction of prolu/prolv for coarse face k along the face direction,
  * mixing in the neighbour across it (3/4, 1/4), zero where the fine face at
  * (2ic, 2jc) is penalised (V6:1794-1798, 1870-1874) */
-__device__ __forceinline__ double corr_u(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
+__device__ __forceinline__ real_t corr_u(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
                                          bool upper)
 {
     const int nxc = C.npx;
     const int ijc = jc * nxc + ic;
     const double f1 = 3. / 4., f2 = 1. / 4.;
-    double c = upper ? f1 * C.cu[ijc] + f2 * C.cu[ijc + nxc] : f1 * C.cu[ijc] + f2 * C.cu[ijc - nxc];
+    real_t c = upper ? f1 * C.cu[ijc] + f2 * C.cu[ijc + nxc] : f1 * C.cu[ijc] + f2 * C.cu[ijc - nxc];
     if (F.flags[(2 * jc) * F.npx + 2 * ic] & MGF_F_SPU) c = 0.0;
     return c;
 }
 
-__device__ __forceinline__ double corr_v(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
+__device__ __forceinline__ real_t corr_v(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
                                          bool right)
 {
     const int nxc = C.npx;
     const int ijc = jc * nxc + ic;
     const double f1 = 3. / 4., f2 = 1. / 4.;
-    double c = right ? f1 * C.cv[ijc] + f2 * C.cv[ijc + 1] : f1 * C.cv[ijc] + f2 * C.cv[ijc - 1];
+    real_t c = right ? f1 * C.cv[ijc] + f2 * C.cv[ijc + 1] : f1 * C.cv[ijc] + f2 * C.cv[ijc - 1];
     if (F.flags[(2 * jc) * F.npx + 2 * ic] & MGF_F_SPV) c = 0.0;
     return c;
 }
@@ -621,7 +621,7 @@ __device__ __forceinline__ double corr_v(const DevLevelG &F, const DevLevelG &C,
 /* bilinear 9/16, 3/16, 3/16, 1/16 correction (V6:1945-1955) of the fine
  * cell on side (di, dj) of a coarse cell, from the coarse correction's 3 x 3
  * neighbourhood n[dj+1][di+1] */
-__device__ __forceinline__ double corr_cell_n(const double n[3][3], int di, int dj)
+__device__ __forceinline__ real_t corr_cell_n(const real_t n[3][3], int di, int dj)
 {
     const double f1 = 9. / 16., f2 = 3. / 16., f3 = 3. / 16., f4 = 1. / 16.;
     return ((f1 * n[1][1] + f2 * n[1][1 + di]) + f3 * n[1 + dj][1]) + f4 * n[1 + dj][1 + di];
@@ -646,10 +646,10 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, cons
      * 2jc-1 */
     {
         const bool hr = ic <= nxc - 3, hl = ic >= 2;
-        double crU = 0.0, crL = 0.0, clU = 0.0, clL = 0.0;
+        real_t crU = 0.0, crL = 0.0, clU = 0.0, clL = 0.0;
         if (hr) { crU = corr_u(F, C, ic, jc, true); crL = corr_u(F, C, ic, jc, false); }
         if (hl) { clU = corr_u(F, C, ic - 1, jc, true); clL = corr_u(F, C, ic - 1, jc, false); }
-        double u = F.u[r2 + i1];
+        real_t u = F.u[r2 + i1];
         if (hl) u = u + 0.5 * clU;
         if (hr) u = u + 0.5 * crU;
         F.u[r2 + i1] = u;
@@ -666,10 +666,10 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, cons
      * 2jc (coarse face jc, if interior); columns 2ic (right) and 2ic-1 */
     {
         const bool hu = jc <= nyc - 3, hd = jc >= 2;
-        double cuR = 0.0, cuL = 0.0, cdR = 0.0, cdL = 0.0;
+        real_t cuR = 0.0, cuL = 0.0, cdR = 0.0, cdL = 0.0;
         if (hu) { cuR = corr_v(F, C, ic, jc, true); cuL = corr_v(F, C, ic, jc, false); }
         if (hd) { cdR = corr_v(F, C, ic, jc - 1, true); cdL = corr_v(F, C, ic, jc - 1, false); }
-        double v = F.v[r1 + i2];
+        real_t v = F.v[r1 + i2];
         if (hd) v = v + 0.5 * cdR;
         if (hu) v = v + 0.5 * cuR;
         F.v[r1 + i2] = v;
@@ -685,7 +685,7 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, cons
     /* cells */
     {
         const int ijc = jc * nxc + ic;
-        double np[3][3], ns[3][3];
+        real_t np[3][3], ns[3][3];
 #pragma unroll
         for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
@@ -720,7 +720,7 @@ __global__ void k_end_cycle(DevCtl *ctls, int n)
     if (ctl->cyc >= ctl->maxcyc) ctl->done = 1;
 }
 
-__global__ void k_fill(double *a, double v, size_t n)
+__global__ void k_fill(real_t *a, double v, size_t n)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) a[k] = v;
@@ -843,7 +843,7 @@ void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st)
     hipLaunchKernelGGL(k_end_cycle, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, ctls, n);
 }
 
-void mgf_launch_fill(double *a, double v, size_t n, hipStream_t st)
+void mgf_launch_fill(real_t *a, double v, size_t n, hipStream_t st)
 {
     if (n == 0) return;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, v, n);

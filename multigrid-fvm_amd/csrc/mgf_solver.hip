@@ -31,12 +31,13 @@
 #include "mgf_dev.h"
 #include "mgf_plan.h"
 #include "mgfvm.h"
+#include "mgfvm_tangent.h"
 
 struct Inst {
     mgf_plan *plan;
     int ng;
     DevLevel lv[MGF_MAX_LEVELS];          /* host copies of the descriptors */
-    double *init_state[MGF_MAX_LEVELS];   /* u, v, p, sc as built at creation */
+    real_t *init_state[MGF_MAX_LEVELS];   /* u, v, p, sc as built at creation */
     int nvcycle;
     double tol;
 };
@@ -112,19 +113,77 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
-    size_t t = 27 * align256(n * sizeof(double));  /* 23 arrays + 4 initial-state copies */
+    size_t t = 27 * align256(n * sizeof(real_t));  /* 23 arrays + 4 initial-state copies */
     if (mixed) t += 10 * align256(n * sizeof(float));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
     t += align256(n * sizeof(uint64_t));            /* sccode */
     t += align256(n * sizeof(int));                 /* pslot */
     t += align256((size_t)(l->npcopy + 1) * sizeof(int));
-    t += align256((size_t)(l->npcopy + 1) * sizeof(double));
+    t += align256((size_t)(l->npcopy + 1) * sizeof(real_t));
     return t;
 }
 
-static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl)
+/* host FP64 values <-> device arrays of the path's scalar: a plain copy when
+ * that scalar is double; for dual numbers the values travel with zero
+ * derivative and come back without it */
+static hipError_t to_device(double *dst, const double *src, size_t n, hipStream_t st)
 {
+    return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st);
+}
+static hipError_t to_device(mgf_dual *dst, const double *src, size_t n, hipStream_t st)
+{
+    std::vector<mgf_dual> h(src, src + n);
+    hipError_t e = hipMemcpyAsync(dst, h.data(), n * sizeof(mgf_dual), hipMemcpyHostToDevice, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
+static hipError_t to_host(double *dst, const double *src, size_t n, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
+static hipError_t to_host(double *dst, const mgf_dual *src, size_t n, hipStream_t st)
+{
+    std::vector<mgf_dual> h(n);
+    hipError_t e = hipMemcpyAsync(h.data(), src, n * sizeof(mgf_dual), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    for (size_t k = 0; k < n; ++k) dst[k] = h[k].v;
+    return e;
+}
+
+/* a value and (in the sensitivity build) its derivative seed */
+static inline void set_real(double &x, double v, double) { x = v; }
+static inline void set_real(mgf_dual &x, double v, double d) { x = mgf_dual(v, d); }
+
+/* derivative seed of u on every level: the inlet column of level 0, carried
+ * to the coarse levels as rest_geom carries the values, and the east
+ * boundary u (rows 1..npy-2 of every level, as init sets it and rest_geom
+ * keeps it) */
+static void inlet_seed(const mgf_plan *pl, const double *d_inlet, double d_u_xp,
+                       std::vector<std::vector<double>> &du)
+{
+    du.assign((size_t)pl->ngrid, std::vector<double>());
+    for (int ig = 0; ig < pl->ngrid; ++ig) {
+        const mgf_plan_level *l = &pl->lv[ig];
+        du[ig].assign((size_t)l->npx * l->npy, 0.0);
+        for (int j = 1; j <= l->npy - 2; ++j) du[ig][(size_t)j * l->npx + l->npx - 2] = d_u_xp;
+    }
+    if (!d_inlet) return;
+    const mgf_plan_level *l0 = &pl->lv[0];
+    for (int j = 1; j <= l0->npy - 2; ++j) du[0][(size_t)j * l0->npx] = d_inlet[j - 1];
+    for (int ig = 1; ig < pl->ngrid; ++ig) {
+        const int nxf = pl->lv[ig - 1].npx, nxc = pl->lv[ig].npx, nyc = pl->lv[ig].npy;
+        for (int jc = 1; jc < nyc - 1; ++jc) {
+            const size_t ij = (size_t)(2 * jc) * nxf;
+            du[ig][(size_t)jc * nxc] = 0.5 * (du[ig - 1][ij] + du[ig - 1][ij - nxf]);
+        }
+    }
+}
+
+static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, const mgf_seed *seed)
+{
+    std::vector<std::vector<double>> du;
+    inlet_seed(I.plan, seed ? seed->d_inlet_u : NULL, seed ? seed->d_u_xp : 0.0, du);
     const mgf_plan *pl = I.plan;
     const mgf_problem *c = &pl->prob;
     auto take = [&](size_t bytes) {
@@ -154,22 +213,27 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl)
         L.ay = l->ay;
         /* the reference evaluates these per point from areax/areay arrays
          * that hold exactly ax/ay; same expressions, same bits */
-        L.du_x = c->amuf * (l->ax / l->dx);
-        L.du_y = c->amuf * (l->ay / l->dy);
-        L.ds_x = amusc * (l->ax / l->dx);
-        L.ds_y = amusc * (l->ay / l->dy);
+        /* d(amusc) = d(amuf) / prsc - amuf d(prsc) / prsc^2 */
+        const double dmu = seed ? seed->d_amuf : 0.0;
+        const double dmusc = seed ? seed->d_amuf / c->prsc -
+                                        c->amuf * seed->d_prsc / (c->prsc * c->prsc)
+                                  : 0.0;
+        set_real(L.du_x, c->amuf * (l->ax / l->dx), dmu * (l->ax / l->dx));
+        set_real(L.du_y, c->amuf * (l->ay / l->dy), dmu * (l->ay / l->dy));
+        set_real(L.ds_x, amusc * (l->ax / l->dx), dmusc * (l->ax / l->dx));
+        set_real(L.ds_y, amusc * (l->ay / l->dy), dmusc * (l->ay / l->dy));
         L.axax = l->ax * l->ax;
         L.ayay = l->ay * l->ay;
         L.bodyx = (c->dpdxm * l->ax) * l->ay;
         L.bodyy = (c->dpdym * l->ax) * l->ay;
         L.P = P;
         L.ctl = ctl;
-        size_t n = (size_t)l->npx * l->npy, nb = n * sizeof(double);
-        double **arrs[23] = {&L.u, &L.v, &L.p, &L.pp, &L.sc, &L.resu, &L.resv, &L.ressc,
+        size_t n = (size_t)l->npx * l->npy, nb = n * sizeof(real_t);
+        real_t **arrs[23] = {&L.u, &L.v, &L.p, &L.pp, &L.sc, &L.resu, &L.resv, &L.ressc,
                              &L.apu, &L.apv, &L.cx, &L.cy, &L.aw, &L.ae, &L.as, &L.an,
                              &L.ap, &L.su, &L.tmp, &L.cu, &L.cv, &L.cp, &L.csc};
-        for (int k = 0; k < 23; ++k) *arrs[k] = (double *)take(nb);
-        I.init_state[ig] = (double *)take(4 * nb);
+        for (int k = 0; k < 23; ++k) *arrs[k] = (real_t *)take(nb);
+        I.init_state[ig] = (real_t *)take(4 * nb);
         L.partial = (double *)take((size_t)L.nblk * sizeof(double));
         L.isolve = c->isolve;
         if (s->opt.precision == MGF_PREC_MIXED) {
@@ -181,14 +245,20 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl)
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
         L.pslot = (const int *)take(n * sizeof(int));
         L.porig = (const int *)take((size_t)(l->npcopy + 1) * sizeof(int));
-        L.pstash = (double *)take((size_t)(l->npcopy + 1) * sizeof(double));
+        L.pstash = (real_t *)take((size_t)(l->npcopy + 1) * sizeof(real_t));
 
         const double *src4[4] = {l->u, l->v, l->p, l->sc};
-        double *dst4[4] = {L.u, L.v, L.p, L.sc};
+        real_t *dst4[4] = {L.u, L.v, L.p, L.sc};
         for (int k = 0; k < 4; ++k) {
-            HIPCHK(s, hipMemcpyAsync(dst4[k], src4[k], nb, hipMemcpyHostToDevice, s->st));
-            HIPCHK(s, hipMemcpyAsync(I.init_state[ig] + k * n, src4[k], nb,
-                                     hipMemcpyHostToDevice, s->st));
+            if (k == 0 && seed && (seed->d_inlet_u || seed->d_u_xp != 0.0)) {
+                std::vector<real_t> h(n);
+                for (size_t q = 0; q < n; ++q) set_real(h[q], l->u[q], du[ig][q]);
+                HIPCHK(s, hipMemcpy(dst4[0], h.data(), nb, hipMemcpyHostToDevice));
+                HIPCHK(s, hipMemcpy(I.init_state[ig], h.data(), nb, hipMemcpyHostToDevice));
+                continue;
+            }
+            HIPCHK(s, to_device(dst4[k], src4[k], n, s->st));
+            HIPCHK(s, to_device(I.init_state[ig] + k * n, src4[k], n, s->st));
         }
         HIPCHK(s, hipMemcpyAsync((void *)L.flags, l->flags, n, hipMemcpyHostToDevice, s->st));
         HIPCHK(s, hipMemcpyAsync((void *)L.sccode, l->sccode, n * sizeof(uint64_t),
@@ -366,7 +436,20 @@ static int ensure_hist(mgf_solver *s, int need)
 extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mgf_options *opt,
                                 mgf_solver **out)
 {
+    return mgf_batch_create_seeded(probs, n, opt, out, NULL);
+}
+
+/* the derivative seeds exist only where the scalar carries derivatives */
+static bool seeds_supported(const double &) { return false; }
+static bool seeds_supported(const mgf_dual &) { return true; }
+
+int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_options *opt,
+                            mgf_solver **out, const mgf_seed *seeds)
+{
     if (!probs || n < 1 || !out) return set_err(NULL, MGF_ERR_ARG, "bad arguments");
+    const real_t probe_scalar = 0.0;
+    if (seeds && !seeds_supported(probe_scalar))
+        return set_err(NULL, MGF_ERR_UNSUPPORTED, "derivative seeds need the sensitivity library");
     *out = NULL;
     mgf_options o;
     if (opt)
@@ -375,6 +458,9 @@ extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mg
         mgf_options_default(&o);
     if (o.precision != MGF_PREC_FP64 && o.precision != MGF_PREC_MIXED)
         return set_err(NULL, MGF_ERR_UNSUPPORTED, "precision mode %d not available", o.precision);
+    if (o.precision == MGF_PREC_MIXED && seeds_supported(probe_scalar))
+        return set_err(NULL, MGF_ERR_UNSUPPORTED,
+                       "the sensitivity library runs the FP64 path only");
     mgf_solver *s = new mgf_solver();
     s->opt = o;
     s->nb = n;
@@ -450,7 +536,8 @@ extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mg
     for (Inst &I : s->inst) maxnv = I.nvcycle > maxnv ? I.nvcycle : maxnv;
     if ((rc = ensure_hist(s, maxnv)) == MGF_OK) {
         char *cur = (char *)s->arena;
-        for (int b = 0; b < n && !rc; ++b) rc = upload_instance(s, s->inst[b], cur, s->d_ctl + b);
+        for (int b = 0; b < n && !rc; ++b)
+            rc = upload_instance(s, s->inst[b], cur, s->d_ctl + b, seeds ? &seeds[b] : NULL);
     }
     if (!rc) rc = build_program(s);
     if (!rc) {
@@ -619,8 +706,8 @@ extern "C" int mgf_reset(mgf_solver *s)
     for (Inst &I : s->inst)
         for (int ig = 0; ig < I.ng; ++ig) {
             DevLevel &L = I.lv[ig];
-            size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(double);
-            double *dst4[4] = {L.u, L.v, L.p, L.sc};
+            size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(real_t);
+            real_t *dst4[4] = {L.u, L.v, L.p, L.sc};
             for (int k = 0; k < 4; ++k)
                 HIPCHK(s, hipMemcpyAsync(dst4[k], I.init_state[ig] + k * n, nb,
                                          hipMemcpyDeviceToDevice, s->st));
@@ -833,11 +920,11 @@ extern "C" int mgf_batch_history(mgf_solver *s, int b, double *history, int cap)
 /* data access                                                              */
 /* ---------------------------------------------------------------------- */
 
-static double *field_ptr(DevLevel &L, const char *name)
+static real_t *field_ptr(DevLevel &L, const char *name)
 {
     struct {
         const char *n;
-        double *p;
+        real_t *p;
     } tab[] = {{"u", L.u},       {"v", L.v},       {"p", L.p},         {"pp", L.pp},
                {"sc", L.sc},     {"resu", L.resu}, {"resv", L.resv},   {"ressc", L.ressc},
                {"apu", L.apu},   {"apv", L.apv}};
@@ -846,16 +933,40 @@ static double *field_ptr(DevLevel &L, const char *name)
     return NULL;
 }
 
+/* device array of a field of problem b, level ig, and its length (for the
+ * sensitivity calls) */
+int mgf_field_array(mgf_solver *s, int b, int ig, const char *name, real_t **ptr, size_t *n,
+                    hipStream_t *st)
+{
+    if (!s || b < 0 || b >= s->nb || ig < 0 || ig >= s->inst[b].ng || !name)
+        return set_err(s, MGF_ERR_ARG, "bad arguments");
+    DevLevel &L = s->inst[b].lv[ig];
+    *ptr = field_ptr(L, name);
+    if (!*ptr) return set_err(s, MGF_ERR_ARG, "unknown field %s", name);
+    *n = (size_t)L.npx * L.npy;
+    *st = s->st;
+    return MGF_OK;
+}
+
+const mgf_problem *mgf_instance_problem(mgf_solver *s, int b)
+{
+    return (s && b >= 0 && b < s->nb) ? &s->inst[b].plan->prob : NULL;
+}
+
+int mgf_set_error(mgf_solver *s, int code, const char *msg)
+{
+    return set_err(s, code, "%s", msg);
+}
+
 extern "C" int mgf_batch_get_field(mgf_solver *s, int b, int ig, const char *name, double *out)
 {
     if (!s || b < 0 || b >= s->nb || ig < 0 || ig >= s->inst[b].ng || !name || !out)
         return set_err(s, MGF_ERR_ARG, "bad arguments");
     DevLevel &L = s->inst[b].lv[ig];
-    double *p = field_ptr(L, name);
+    real_t *p = field_ptr(L, name);
     if (!p) return set_err(s, MGF_ERR_ARG, "unknown field %s", name);
     const size_t n = (size_t)L.npx * L.npy;
-    HIPCHK(s, hipMemcpyAsync(out, p, n * sizeof(double), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(s, hipStreamSynchronize(s->st));
+    HIPCHK(s, to_host(out, p, n, s->st));
     return MGF_OK;
 }
 
@@ -869,10 +980,10 @@ extern "C" int mgf_set_field(mgf_solver *s, int ig, const char *name, const doub
     if (!s || ig < 0 || ig >= s->inst[0].ng || !name || !in)
         return set_err(s, MGF_ERR_ARG, "bad arguments");
     DevLevel &L = s->inst[0].lv[ig];
-    double *p = field_ptr(L, name);
+    real_t *p = field_ptr(L, name);
     if (!p) return set_err(s, MGF_ERR_ARG, "unknown field %s", name);
     const size_t n = (size_t)L.npx * L.npy;
-    HIPCHK(s, hipMemcpyAsync(p, in, n * sizeof(double), hipMemcpyHostToDevice, s->st));
+    HIPCHK(s, to_device(p, in, n, s->st));
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
 }
@@ -921,13 +1032,13 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     const int field = (kernel == MGF_KERNEL_SWEEP_PP || kernel == MGF_KERNEL_RELAX_PP) ? 2 : 0;
     const int isolve = s->isolve == 1 ? 1 : 2;
     const int even = reps + (reps & 1);    /* an even count ends without a copy-back */
-    double *save = NULL, *phi = NULL;
+    real_t *save = NULL, *phi = NULL;
     const size_t n = (size_t)L0.npx * L0.npy;
     if (!mixed) {
         /* FP64 sweeps work in place: keep the field */
         phi = field == 2 ? L0.pp : L0.u;
-        HIPCHK(s, hipMalloc(&save, n * sizeof(double)));
-        HIPCHK(s, hipMemcpyAsync(save, phi, n * sizeof(double), hipMemcpyDeviceToDevice, s->st));
+        HIPCHK(s, hipMalloc(&save, n * sizeof(real_t)));
+        HIPCHK(s, hipMemcpyAsync(save, phi, n * sizeof(real_t), hipMemcpyDeviceToDevice, s->st));
     }
     /* the sweeps check the stop flags: clear them for the measurement */
     for (int b = 0; b < s->nb; ++b) s->h_ctl[b].done = 0;
@@ -965,7 +1076,7 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (save) {
-        (void)hipMemcpyAsync(phi, save, n * sizeof(double), hipMemcpyDeviceToDevice, s->st);
+        (void)hipMemcpyAsync(phi, save, n * sizeof(real_t), hipMemcpyDeviceToDevice, s->st);
         (void)hipStreamSynchronize(s->st);
         (void)hipFree(save);
     }

@@ -16,24 +16,24 @@
 #define BY 4
 
 struct Coef {
-    double aw, ae, as, an, ap;
+    real_t aw, ae, as, an, ap;
 };
 
 /* face fluxes from the materialised arrays (relaxation: cx/cy are computed
  * once at its start, V6:201, and the u/v/sc assemblies read those) */
 struct FluxArr {
     const gdbl *cx, *cy;
-    __device__ double X(int k) const { return cx[k]; }
-    __device__ double Y(int k) const { return cy[k]; }
+    __device__ real_t X(int k) const { return cx[k]; }
+    __device__ real_t Y(int k) const { return cy[k]; }
 };
 /* face fluxes recomputed from the current velocities (restriction: the
  * reference recomputes them right before, V6:215-216, with nothing changing
  * u/v in between), same expression as fluxes (V6:690-691) */
 struct FluxInl {
     const gdbl *u, *v;
-    double ax, ay;
-    __device__ double X(int k) const { return (0.5 * u[k]) * ax; }
-    __device__ double Y(int k) const { return (0.5 * v[k]) * ay; }
+    real_t ax, ay;
+    __device__ real_t X(int k) const { return (0.5 * u[k]) * ax; }
+    __device__ real_t Y(int k) const { return (0.5 * v[k]) * ay; }
 };
 
 /* Workgroups of a launch go to the 8 XCDs round-robin by linear id, and
@@ -87,13 +87,13 @@ static inline dim3 tile_grid(int ni, int nj, int nz, int by = BY)
 /* coefu + no-slip wall doubling at one u face in row j (V6:825-858) from
  * the face fluxes around it: cx at (i-1, i, i+1, row j), cy at (i, i+1) of
  * rows j-1 and j */
-__device__ __forceinline__ Coef coefu_vals(const DevLevelG &L, int j, double xw, double x0,
-                                           double xe, double ys0, double ys1, double y0, double y1)
+__device__ __forceinline__ Coef coefu_vals(const DevLevelG &L, int j, real_t xw, real_t x0,
+                                           real_t xe, real_t ys0, real_t ys1, real_t y0, real_t y1)
 {
-    double cxm = 0.5 * (x0 + xw);
-    double cxp = 0.5 * (x0 + xe);
-    double cym = 0.5 * (ys0 + ys1);
-    double cyp = 0.5 * (y0 + y1);
+    real_t cxm = 0.5 * (x0 + xw);
+    real_t cxp = 0.5 * (x0 + xe);
+    real_t cym = 0.5 * (ys0 + ys1);
+    real_t cyp = 0.5 * (y0 + y1);
     Coef c;
     c.aw = fmax(fabs(cxm), L.du_x) + cxm;
     c.ae = fmax(fabs(cxp), L.du_x) - cxp;
@@ -116,14 +116,14 @@ __device__ __forceinline__ Coef coefu_at(const DevLevelG &L, const FX &f, int i,
 
 /* coefv + side-wall adds at one v face in column i (V6:976-1015) from cx
  * at (i-1, i) of rows j and j+1 and cy at rows j-1, j, j+1 of column i */
-__device__ __forceinline__ Coef coefv_vals(const DevLevelG &L, int i, double xw, double xwn,
-                                           double x0, double xn, double ys, double y0, double yn)
+__device__ __forceinline__ Coef coefv_vals(const DevLevelG &L, int i, real_t xw, real_t xwn,
+                                           real_t x0, real_t xn, real_t ys, real_t y0, real_t yn)
 {
     const int nx = L.npx;
-    double cxm = 0.5 * (xw + xwn);
-    double cxp = 0.5 * (x0 + xn);
-    double cym = 0.5 * (y0 + ys);
-    double cyp = 0.5 * (y0 + yn);
+    real_t cxm = 0.5 * (xw + xwn);
+    real_t cxp = 0.5 * (x0 + xn);
+    real_t cym = 0.5 * (y0 + ys);
+    real_t cyp = 0.5 * (y0 + yn);
     Coef c;
     c.aw = fmax(fabs(cxm), L.du_x) + cxm;
     c.ae = fmax(fabs(cxp), L.du_x) - cxp;
@@ -147,8 +147,8 @@ __device__ __forceinline__ Coef coefv_at(const DevLevelG &L, const FX &f, int i,
 /* coefsc at cell ij including the obstacle diffusion adds that the
  * reference applies on the host in obstacle-list order (V6:872-963), from
  * the fluxes through its four faces */
-__device__ __forceinline__ Coef coefsc_vals(const DevLevelG &L, int ij, uint8_t fl, double cxm,
-                                            double cxp, double cym, double cyp)
+__device__ __forceinline__ Coef coefsc_vals(const DevLevelG &L, int ij, uint8_t fl, real_t cxm,
+                                            real_t cxp, real_t cym, real_t cyp)
 {
     Coef c;
     c.aw = fmax(fabs(cxm), L.ds_x) + cxm;

@@ -138,18 +138,42 @@ def free_problem(p):
     lib().mgf_problem_free(p)
 
 
+class InletProfile:
+    """Attach an inlet u profile (nfy values, rows 1..nfy of the level-0 west
+    face) to a problem read by the library, for as long as the `with` block
+    runs; the problem never owns the buffer."""
+
+    def __init__(self, prob, inlet_u):
+        self.prob = prob
+        self.buf = None
+        if inlet_u is not None:
+            a = np.ascontiguousarray(inlet_u, dtype=np.float64)
+            if a.size != prob.contents.nfy:
+                raise ValueError(f"inlet profile needs {prob.contents.nfy} values")
+            self.buf = (ctypes.c_double * a.size)(*a.tolist())
+
+    def __enter__(self):
+        if self.buf is not None:
+            self.prob.contents.inlet_u = ctypes.cast(self.buf, ctypes.POINTER(ctypes.c_double))
+        return self.prob
+
+    def __exit__(self, *a):
+        self.prob.contents.inlet_u = ctypes.POINTER(ctypes.c_double)()
+
+
 class Solver:
     """One problem resident on one GPU."""
 
     def __init__(self, inputdata: str, pattern: str, precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, strict_limits: bool = False,
                  sweeps_per_launch: int = 0, sweep_threads: int = 0,
-                 split_assembly: bool = False):
+                 split_assembly: bool = False, inlet_u=None):
         L = lib()
         prob = read_problem(inputdata, pattern)
         try:
-            self._init_from(prob, precision, use_graph, device, strict_limits, sweeps_per_launch,
-                            sweep_threads, split_assembly)
+            with InletProfile(prob, inlet_u):
+                self._init_from(prob, precision, use_graph, device, strict_limits,
+                                sweeps_per_launch, sweep_threads, split_assembly)
         finally:
             free_problem(prob)
 

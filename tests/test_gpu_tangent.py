@@ -1,0 +1,116 @@
+"""Forward sensitivities (libmgfvm_tangent.so): the value fields equal the
+FP64 library's bit for bit, and the derivatives of the objectives along
+amuf, prsc and single inlet rows agree with central finite differences of
+FP64 solves (tolerance 1e-5 relative, the gradient check the benchmark's
+differentiation workload states)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from mgfvm.inputs import Problem, Rng, random_mask
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _problem(u_xp):
+    """64x64 channel, 16x16 obstacle base (Bernoulli porosity 0.85), Re 100,
+    red-black 2/6, plug east boundary at the mean inlet velocity (so inflow
+    and outflow balance), converged to sum|div| < 1e-12."""
+    return Problem(nfx=64, nfy=64, ngrid=3, ig_obs=3, nvcycle=3000, convergence=1e-12,
+                   isolve=2, nswpm=2, nswpp=6, amuf=0.01, bc_xp=(u_xp, 0.0, 0.0, 0.0),
+                   obstacles=random_mask(16, 0.85, 11))
+
+
+def _inlet(nfy, seed=3):
+    y = (np.arange(nfy) + 0.5) / nfy
+    rng = Rng(seed)
+    return 6.0 * y * (1.0 - y) + np.array([0.05 * rng.normal() for _ in range(nfy)])
+
+
+def _objectives(s, yl):
+    u, p, sc = s.get(0, "u"), s.get(0, "p"), s.get(0, "sc")
+    npy, npx = u.shape
+    ax = yl / (npy - 2)
+    j1 = p[1:-1, 1].sum() / (npy - 2) - p[1:-1, npx - 2].sum() / (npy - 2)
+    j2 = 0.0
+    for j in range(1, npy - 1):
+        j2 += (ax * u[j, npx - 2]) * sc[j, npx - 2]
+    return np.array([j1, j2])
+
+
+def _fp64_J(tmp_path, prob, inlet, tag):
+    import mgfvm
+    pin, ppat = prob.write(str(tmp_path / tag))
+    s = mgfvm.Solver(pin, ppat, inlet_u=inlet)
+    _, n, conv = s.solve()
+    assert conv, tag
+    J = _objectives(s, prob.yl)
+    s.close()
+    return J
+
+
+def test_tangent_values_and_finite_differences(tmp_path):
+    import mgfvm
+    from mgfvm.tangent import TangentBatch
+    inlet = _inlet(64)
+    prob = _problem(float(inlet.mean()))
+    pin, ppat = prob.write(str(tmp_path / "base"))
+    rows = [5, 20, 40]
+    seeds = [{"d_amuf": 1.0}, {"d_prsc": 1.0}]
+    for r in rows:
+        e = np.zeros(prob.nfy)
+        e[r - 1] = 1.0
+        seeds.append({"d_inlet_u": e, "d_u_xp": 1.0 / prob.nfy})
+    _say(f"[gpu] tangent batch of {len(seeds)}")
+    tb = TangentBatch([(pin, ppat)] * len(seeds), seeds, inlet_u=inlet)
+    ncyc, conv = tb.solve()
+    assert all(conv) and len(set(ncyc)) == 1
+
+    s = mgfvm.Solver(pin, ppat, inlet_u=inlet)
+    _, n, c = s.solve()
+    assert c and n == ncyc[0]
+    for name in ("u", "v", "p", "sc"):
+        v, _ = tb.get(0, 0, name)
+        assert np.array_equal(v.view(np.uint64), s.get(0, name).view(np.uint64)), name
+    J0 = _objectives(s, prob.yl)
+    s.close()
+    Jt, _ = tb.objectives(0)
+    np.testing.assert_allclose(Jt, J0, rtol=1e-13)
+
+    # central steps: small enough relative to each input that no face of the
+    # hybrid scheme changes branch (|F| vs amuf * area / spacing), large
+    # enough that the round-off of J (~1e-13 relative) stays below 1e-6
+    checks = []
+    for k, sd in enumerate(seeds):
+        _, dJ = tb.objectives(k)
+        h = 1e-8 if "d_amuf" in sd else (1e-6 if "d_prsc" in sd else 1e-5)
+        if "d_amuf" in sd:
+            jp = _fp64_J(tmp_path, prob.replace(amuf=prob.amuf + h), inlet, f"amuf_p")
+            jm = _fp64_J(tmp_path, prob.replace(amuf=prob.amuf - h), inlet, f"amuf_m")
+        elif "d_prsc" in sd:
+            jp = _fp64_J(tmp_path, prob.replace(prsc=prob.prsc + h), inlet, f"prsc_p")
+            jm = _fp64_J(tmp_path, prob.replace(prsc=prob.prsc - h), inlet, f"prsc_m")
+        else:
+            e = sd["d_inlet_u"]
+            up = (prob.bc_xp[0] + h / prob.nfy, 0.0, 0.0, 0.0)
+            um = (prob.bc_xp[0] - h / prob.nfy, 0.0, 0.0, 0.0)
+            jp = _fp64_J(tmp_path, prob.replace(bc_xp=up), inlet + h * e, f"in{k}_p")
+            jm = _fp64_J(tmp_path, prob.replace(bc_xp=um), inlet - h * e, f"in{k}_m")
+        fd = (jp - jm) / (2 * h)
+        checks.append((k, dJ, fd))
+        _say(f"[gpu] direction {k}: tangent {dJ} fd {fd}")
+    tb.close()
+    for k, dJ, fd in checks:
+        for q in range(2):
+            if abs(fd[q]) < 1e-9 and abs(dJ[q]) < 1e-9:
+                continue          # both zero (J1 does not depend on prsc)
+            assert abs(dJ[q] - fd[q]) <= REL_TOL * max(abs(fd[q]), 1e-12), (k, q, dJ[q], fd[q])
