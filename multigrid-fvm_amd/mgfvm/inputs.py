@@ -256,6 +256,23 @@ def random_1024(ngrid: int = 7, seed: int = 1, porosity: float = 0.8) -> Problem
                    nswpm=2, nswpp=6, isolve=2, amuf=0.005, obstacles=rects)
 
 
+def sensitivity_256(seed: int = 1):
+    """Differentiation workload: 256x256 channel, 5 levels, Bernoulli(0.85)
+    mask on a 32x32 base grid (8x8 fine cells per base cell), Re 100,
+    red-black 2/6, converged to sum|div| < 1e-12; inlet u profile parabolic
+    (mean 1) plus seeded N(0, 0.05) noise, east boundary u at its mean.
+    Returns (problem, inlet profile)."""
+    nfy = 256
+    y = [(j + 0.5) / nfy for j in range(nfy)]
+    rng = Rng(seed + 7919)
+    inlet = [6.0 * t * (1.0 - t) + 0.05 * rng.normal() for t in y]
+    mean = sum(inlet) / nfy
+    p = Problem(nfx=256, nfy=nfy, ngrid=5, ig_obs=4, nvcycle=3000, convergence=1e-12,
+                nswpm=2, nswpp=6, isolve=2, amuf=0.01, bc_xp=(mean, 0.0, 0.0, 0.0),
+                obstacles=random_mask(32, 0.85, seed))
+    return p, inlet
+
+
 def parse(inputdata: str, pattern: str) -> Problem:
     """Read an InputData / pattern.txt pair (whitespace separated, in the
     reference's reading order)."""

@@ -174,3 +174,52 @@ def gradients(inputdata: str, pattern: str, inlet_u=None, rows: Optional[Sequenc
     J = outs[0][0]
     dJ = np.array([o[1] for o in outs])            # directions x 2
     return J, dJ[:len(rows)].T, dJ[len(rows)], dJ[len(rows) + 1], all(conv)
+
+
+def _problem_files(problem, inlet_u, amuf, prsc, directory):
+    """Write the problem with the given amuf, prsc and an east boundary u
+    equal to the mean inlet velocity (inflow = outflow)."""
+    bc = tuple(problem.bc_xp)
+    p = problem.replace(amuf=float(amuf), prsc=float(prsc),
+                        bc_xp=(float(np.mean(inlet_u)), bc[1], bc[2], bc[3]))
+    return p.write(directory)
+
+
+def steady_objectives(inlet_u, amuf, prsc, problem, device: int = -1):
+    """J = (J1, J2) of `problem` (an inputs.Problem) with the given inlet
+    profile, amuf and prsc, and its gradient, from one batched tangent
+    solve.  Returns (J, dJ_dinlet [2 x nfy], dJ_damuf [2], dJ_dprsc [2],
+    converged)."""
+    import tempfile
+    with tempfile.TemporaryDirectory(prefix="mgf_tan_") as d:
+        pin, ppat = _problem_files(problem, inlet_u, amuf, prsc, d)
+        return gradients(pin, ppat, inlet_u=np.asarray(inlet_u, dtype=np.float64), device=device)
+
+
+def torch_function():
+    """The torch.autograd.Function over steady_objectives: forward(inlet_u
+    [nfy], amuf, prsc, problem) -> J [2] (float64); backward returns the
+    vector-Jacobian products from the forward-mode Jacobian computed in the
+    same solve."""
+    import torch
+
+    class SteadyObjectives(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, inlet_u, amuf, prsc, problem):
+            J, d_in, d_mu, d_pr, conv = steady_objectives(
+                inlet_u.detach().cpu().double().numpy(), float(amuf), float(prsc), problem)
+            if not conv:
+                raise RuntimeError("steady solve did not converge")
+            ctx.save_for_backward(torch.as_tensor(d_in), torch.as_tensor(d_mu),
+                                  torch.as_tensor(d_pr))
+            ctx.devices = (inlet_u.device,)
+            return torch.as_tensor(J, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, g):
+            d_in, d_mu, d_pr = ctx.saved_tensors
+            g = g.to(torch.float64).cpu()
+            return ((g @ d_in).to(ctx.devices[0]), (g @ d_mu).reshape(()),
+                    (g @ d_pr).reshape(()), None)
+
+    return SteadyObjectives

@@ -134,3 +134,16 @@ def test_driver_reports_input_errors(tmp_path):
     assert os.path.exists(exe)
     r = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True)
     assert r.returncode == 1 and "InputData" in r.stderr
+
+
+def test_sensitivity_library_exports():
+    """libmgfvm_tangent.so exports the whole mgfvm.h ABI and the calls of
+    mgfvm_tangent.h."""
+    from mgfvm import tangent
+    L = tangent.tlib()
+    txt = open(os.path.join(ROOT, "include", "mgfvm_tangent.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    extra = sorted(set(re.findall(r"\b(mgf_[a-z_0-9]+)\s*\(", txt)))
+    assert extra == sorted(tangent.EXPORTS)
+    missing = [n for n in declared() + extra if not hasattr(L, n)]
+    assert not missing, missing

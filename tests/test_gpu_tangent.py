@@ -114,3 +114,66 @@ def test_tangent_values_and_finite_differences(tmp_path):
             if abs(fd[q]) < 1e-9 and abs(dJ[q]) < 1e-9:
                 continue          # both zero (J1 does not depend on prsc)
             assert abs(dJ[q] - fd[q]) <= REL_TOL * max(abs(fd[q]), 1e-12), (k, q, dJ[q], fd[q])
+
+
+def _fd_case(tmp_path, prob, inlet, kind, h, e=None):
+    if kind == "amuf":
+        a, b = prob.replace(amuf=prob.amuf + h), prob.replace(amuf=prob.amuf - h)
+        ia = ib = inlet
+    elif kind == "prsc":
+        a, b = prob.replace(prsc=prob.prsc + h), prob.replace(prsc=prob.prsc - h)
+        ia = ib = inlet
+    else:
+        a = prob.replace(bc_xp=(prob.bc_xp[0] + h / prob.nfy, 0.0, 0.0, 0.0))
+        b = prob.replace(bc_xp=(prob.bc_xp[0] - h / prob.nfy, 0.0, 0.0, 0.0))
+        ia, ib = inlet + h * e, inlet - h * e
+    return (_fp64_J(tmp_path, a, ia, f"{kind}_p") - _fp64_J(tmp_path, b, ib, f"{kind}_m")) / (2 * h)
+
+
+def test_gradients_256_against_finite_differences(tmp_path):
+    """The differentiation workload at full size (256x256, 5 levels,
+    porosity-0.85 mask on a 32x32 base, Re 100, noisy parabolic inlet):
+    mgfvm.tangent.gradients for inlet rows 1 and 128, amuf and prsc against
+    central differences of FP64 solves."""
+    from mgfvm.inputs import sensitivity_256
+    from mgfvm.tangent import gradients
+    prob, inlet = sensitivity_256(1)
+    inlet = np.array(inlet)
+    pin, ppat = prob.write(str(tmp_path / "base"))
+    _say("[gpu] gradients 256x256")
+    J, din, dmu, dpr, conv = gradients(pin, ppat, inlet_u=inlet, rows=[1, 128])
+    assert conv
+    cases = [("amuf", 1e-8, None, dmu), ("prsc", 1e-6, None, dpr)]
+    for k, r in enumerate((1, 128)):
+        e = np.zeros(prob.nfy)
+        e[r - 1] = 1.0
+        cases.append(("inlet", 1e-5, e, din[:, k]))
+    for kind, h, e, d in cases:
+        fd = _fd_case(tmp_path, prob, inlet, kind, h, e)
+        _say(f"[gpu] 256^2 {kind}: tangent {d} fd {fd}")
+        for q in range(2):
+            if abs(fd[q]) < 1e-9 and abs(d[q]) < 1e-9:
+                continue
+            assert abs(d[q] - fd[q]) <= REL_TOL * abs(fd[q]), (kind, q, d[q], fd[q])
+
+
+def test_torch_function_backward(tmp_path):
+    """torch.autograd over the steady objectives: backward gives g . dJ/dtheta
+    with the forward-mode Jacobian of the same solve."""
+    import torch
+    from mgfvm.tangent import steady_objectives, torch_function
+    inlet = _inlet(64)
+    prob = _problem(float(inlet.mean()))
+    F = torch_function()
+    x = torch.tensor(inlet, dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor(prob.amuf, dtype=torch.float64, requires_grad=True)
+    pr = torch.tensor(prob.prsc, dtype=torch.float64, requires_grad=True)
+    _say("[gpu] torch autograd over the steady objectives")
+    J = F.apply(x, mu, pr, prob)
+    (J[0] + 3.0 * J[1]).backward()
+    J2, din, dmu, dpr, conv = steady_objectives(inlet, prob.amuf, prob.prsc, prob)
+    np.testing.assert_allclose(J.detach().numpy(), J2, rtol=0, atol=0)
+    w = np.array([1.0, 3.0])
+    np.testing.assert_allclose(x.grad.numpy(), w @ din, rtol=1e-14)
+    np.testing.assert_allclose(mu.grad.item(), w @ dmu, rtol=1e-14)
+    np.testing.assert_allclose(pr.grad.item(), w @ dpr, rtol=1e-14)
