@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-single", action="store_true", help="skip the 1024^2 single solve")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-at-a-time figure")
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the 256^2 gradient (forward sensitivities) figure")
     ap.add_argument("--reps", type=int, default=100, help="launches timed for the roofline")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the batched V-cycles eagerly instead of replaying a hipGraph")
@@ -291,6 +293,37 @@ def main():
                     single["fp64_sweep_GBps"] = round(b1 / (us1 * 1e-6) / 1e9, 1)
                 s.close()
 
+        # ---- gradient of the 256^2 differentiation workload -----------------
+        gradient = None
+        if not args.no_gradient:
+            import numpy as np
+            from mgfvm.inputs import sensitivity_256
+            from mgfvm.tangent import gradients
+            pg, inlet = sensitivity_256(1)
+            gin, gpat = pg.write(os.path.join(work, "grad256"))
+            s = mgfvm.Solver(gin, gpat, inlet_u=np.array(inlet), device=0)
+            s.solve()
+            s.reset()
+            _sync(s)
+            t0 = time.perf_counter()
+            _, ncg, cg = s.solve()
+            _sync(s)
+            t_primal = time.perf_counter() - t0
+            s.close()
+            t0 = time.perf_counter()
+            J, din, dmu, dpr, cgrad = gradients(gin, gpat, inlet_u=np.array(inlet), device=0)
+            t_grad = time.perf_counter() - t0
+            gradient = {"workload": "256x256 channel, 5 levels, Bernoulli(0.85) mask on a 32x32 "
+                                    "base, Re 100, noisy parabolic inlet, FP64 to sum|div| < 1e-12",
+                        "directions": din.shape[1] + 2, "primal_s": round(t_primal, 3),
+                        "primal_v_cycles": ncg, "gradient_s": round(t_grad, 3),
+                        "central_fd_equivalent_s": round(2 * (din.shape[1] + 2) * t_primal, 1),
+                        "converged": bool(cg and cgrad),
+                        "J": [float(x) for x in J], "dJ_damuf": [float(x) for x in dmu],
+                        "dJ_dprsc": [float(x) for x in dpr]}
+            print(f"[bench] gradient 256^2: {t_grad:.2f}s for {din.shape[1] + 2} directions",
+                  file=sys.stderr, flush=True)
+
         # ---- CPU baseline ---------------------------------------------------
         cpu = None
         if not args.no_cpu:
@@ -329,7 +362,8 @@ def main():
                        "plain_fp64_gpu_estimate": plain_detail,
                        "batched_mixed_vs_plain":
                            round(value / plain_value, 2) if plain_value else None,
-                       "single_1024": single},
+                       "single_1024": single,
+                       "gradient_256": gradient},
             "roofline": roof, "cpu_baseline": cpu,
         }
         print(json.dumps(line))
