@@ -98,6 +98,11 @@ typedef struct mgf_options {
                                   a kernel of its own that stores it in FP32; 0 (default)
                                   = the first sweep launch assembles it on the fly.
                                   Result-neutral. */
+    int compact;               /* batched solves with use_graph: once enough instances
+                                  have stopped, capture the V-cycle again over the live
+                                  ones only (1, default), instead of launching over all
+                                  of them with the stopped ones returning at once (0).
+                                  Result-neutral. */
 } mgf_options;
 
 typedef struct mgf_solver mgf_solver;

@@ -31,7 +31,9 @@ struct DevCtl {
     int maxcyc;              /* cycle limit of this solve */
     double tol;
     double *hist;            /* residual history of this instance */
-    double err[MGF_MAX_LEVELS];   /* latest sum |div| per level */
+    double err[MGF_MAX_LEVELS];   /* latest sum |div| per level, from the relaxations
+                                     that reduce it (the step calls always; a V-cycle
+                                     only on the recorded level-0 relaxation) */
 };
 
 struct DevPhys {
@@ -84,8 +86,10 @@ struct DevPhys {
     /* mixed precision (null in FP64 mode): FP32 coefficients, 1/diagonal,           \
      * right-hand side (the FP64 residual for u, v, sc; -div for p'), the two        \
      * buffers of the FP32 correction sweeps, and the corrections of u and v,        \
-     * which stay pending (u + du, v + dv) until the update writes u and v */         \
+     * which stay pending (u + du, v + dv) until the update writes u and v, and      \
+     * (when u, v and sc relax in one launch) that of sc, applied there too */        \
     GQ float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32, *du32, *dv32;    \
+    GQ float *ds32;                                                                   \
     /* obstacle data */                                                               \
     GQ const uint8_t *flags;                                                          \
     GQ const uint64_t *sccode;                                                        \
@@ -113,6 +117,7 @@ struct DevGroup {
     int npx, npy;
     int max_npcopy;
     int any_pbound_x, any_pbound_y, any_outflow;
+    int any_finest;          /* some member is its instance's level 0 */
     int depth;               /* levels above the coarsest */
     int up;                  /* 1: the up-leg subset (no finest levels) */
 };
@@ -140,11 +145,14 @@ int mgf_set_error(mgf_solver *s, int code, const char *msg);
 void mgf_launch_fluxes(const DevGroup &G, hipStream_t st);
 void mgf_launch_asm_u(const DevGroup &G, hipStream_t st);
 void mgf_launch_asm_v(const DevGroup &G, hipStream_t st);
-void mgf_launch_asm_p(const DevGroup &G, int record, hipStream_t st);
+/* err: reduce the sum |div| (1), and record it as the history entry of
+ * level-0 members (2), or neither (0) */
+void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st);
 void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
 /* field: 0 u, 1 v, 2 pp, 3 sc */
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
-/* pending: mixed precision, u and v corrections still in du32 / dv32 */
+/* pending: mixed precision, u and v corrections still in du32 / dv32 (1),
+ * and the sc correction in ds32 as well (2) */
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);
@@ -152,7 +160,7 @@ void mgf_launch_restrict(const DevPair &R, hipStream_t st);
 void mgf_launch_prolong(const DevPair &R, hipStream_t st);
 void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
-void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st);
+void mgf_launch_asm_mixed(const DevGroup &G, int field, int err, hipStream_t st);
 /* shape of the temporally blocked red-black sweeps: sweeps per launch,
  * threads per workgroup of the launches on the stored equation (nt) and of
  * the assembling launches (nt_asm) -- each workgroup takes a 64 x
@@ -175,7 +183,14 @@ double mgf_launch_relax_first_mixed(const DevGroup &G, int field, int nsw, Sweep
                                     int *n_out, hipStream_t st);
 /* assembly + sweeps + FP64 update of one field in a relaxation */
 void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int nsw,
-                                  SweepCfg cfg, int record, hipStream_t st);
+                                  SweepCfg cfg, int err, hipStream_t st);
+/* u, v and sc of a relaxation in one launch, sc's correction left pending
+ * for mgf_launch_update(G, 2); 0 (nothing launched) when the group or the
+ * configuration does not allow it: red-black with fused assembly and nsw
+ * within one launch, and no outflow boundary (which rewrites sc's ghost
+ * between the update and the sc solve) */
+int mgf_launch_relax_uvs_mixed(const DevGroup &G, int isolve, int nsw, SweepCfg cfg,
+                               hipStream_t st);
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
                              hipStream_t st);
 /* the FP32 sweeps alone (they only touch the correction buffers); returns

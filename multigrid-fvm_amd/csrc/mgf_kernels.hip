@@ -337,10 +337,12 @@ __global__ void k_pstash(const DevLevel *Ls)
     L.pstash[s] = in ? L.p[o] + P.relxp * (L.pp[o] - 0.0) : L.p[o];
 }
 
-/* `pending`: mixed precision, where the corrections of the u and v solves
- * are still in du32 / dv32 over their swept ranges (the value of u there is
- * u + du, formed exactly as the solve's own update would) */
-template <bool PENDING>
+/* PENDING >= 1: mixed precision, where the corrections of the u and v
+ * solves are still in du32 / dv32 over their swept ranges (the value of u
+ * there is u + du, formed exactly as the solve's own update would); 2: the
+ * sc solve ran in the same launch as u and v, and its correction in ds32 is
+ * applied here (sc += d, as k_apply_mx would; the update does not read sc) */
+template <int PENDING>
 __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
 {
     LEVEL(Ls);
@@ -357,6 +359,7 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
     }
     L.u[ij] = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
     L.v[ij] = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
+    if (PENDING == 2) L.sc[ij] = L.sc[ij] + (double)L.ds32[ij];
     if (L.flags[ij] & MGF_F_PCOPY)
         L.p[ij] = L.pstash[L.pslot[ij]];
     else
@@ -760,10 +763,10 @@ void mgf_launch_reduce_err(const DevGroup &G, int record, hipStream_t st)
                        record);
 }
 
-void mgf_launch_asm_p(const DevGroup &G, int record, hipStream_t st)
+void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st)
 {
     hipLaunchKernelGGL(k_asm_p, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
-    mgf_launch_reduce_err(G, record, st);
+    if (err) mgf_launch_reduce_err(G, err == 2, st);
 }
 
 void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st)
@@ -799,12 +802,13 @@ void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st)
         hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
-    if (pending)
-        hipLaunchKernelGGL(k_update<true>, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY),
-                           0, st, G.d);
+    const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
+    if (pending == 2)
+        hipLaunchKernelGGL(k_update<2>, g, dim3(BX, BY), 0, st, G.d);
+    else if (pending)
+        hipLaunchKernelGGL(k_update<1>, g, dim3(BX, BY), 0, st, G.d);
     else
-        hipLaunchKernelGGL(k_update<false>, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY),
-                           0, st, G.d);
+        hipLaunchKernelGGL(k_update<0>, g, dim3(BX, BY), 0, st, G.d);
 }
 
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st)

@@ -43,7 +43,23 @@ __device__ __forceinline__ void store_pt(const DevLevelG &L, int g, const Pt32 &
 /* FP64 assembly of one equation at one point plus its FP64 residual, rounded
  * to FP32 (field 0 u, 1 v, 3 sc); ap_out = the FP64 diagonal, which the
  * caller stores as apu / apv for u / v.  No global stores here, so a caller
- * assembling many points can keep all their loads in flight. */
+ * assembling many points can keep all their loads in flight.  INL: the face
+ * fluxes are formed from u and v (FluxInl) instead of read from cx / cy;
+ * the same values while u and v are those cx / cy were computed from. */
+template <bool INL>
+__device__ __forceinline__ Coef coef_pt(const DevLevelG &L, int field, int i, int j, uint8_t fl)
+{
+    if (INL) {
+        const FluxInl f{L.u, L.v, L.ax, L.ay};
+        return field == 0 ? coefu_at(L, f, i, j)
+                          : (field == 1 ? coefv_at(L, f, i, j) : coefsc_at(L, f, i, j, fl));
+    }
+    const FluxArr f{L.cx, L.cy};
+    return field == 0 ? coefu_at(L, f, i, j)
+                      : (field == 1 ? coefv_at(L, f, i, j) : coefsc_at(L, f, i, j, fl));
+}
+
+template <bool INL = false>
 __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int j, double &ap_out)
 {
     const DevPhys &P = L.P;
@@ -53,14 +69,14 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int
     double su, ap;
     const gdbl *phi;
     if (field == 0) {
-        c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
+        c = coef_pt<INL>(L, 0, i, j, 0);
         su = L.ax * (L.p[ij] - L.p[ij + 1]);
         ap = c.ap / P.relxm;
         su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
         ap = ap + ((L.flags[ij] & MGF_F_SPU) ? 1.0e30 : 0.0);
         phi = L.u;
     } else if (field == 1) {
-        c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
+        c = coef_pt<INL>(L, 1, i, j, 0);
         su = L.ay * (L.p[ij] - L.p[ij + nx]);
         ap = c.ap / P.relxm;
         su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
@@ -68,7 +84,7 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int
         phi = L.v;
     } else {
         const uint8_t fl = L.flags[ij];
-        c = coefsc_at(L, FluxArr{L.cx, L.cy}, i, j, fl);
+        c = coef_pt<INL>(L, 3, i, j, fl);
         const double spsc = (fl & MGF_F_SPSC) ? 1.0e30 : 0.0;
         su = spsc * P.sc_obs + L.ressc[ij];
         ap = c.ap + spsc;
@@ -170,40 +186,54 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
  * and, for p', the |div| partial sums (per 64 x 4 block, in block_sum's
  * order, so they equal k_asm_p_mx's bit for bit) are written as by the
  * assembly kernels.  MODE_ASM_STORE: MODE_ASM that also stores the tile's
- * FP32 equation for the launches that follow.  Nothing the assembly reads is
- * written by the launch, so halo recomputation is race-free. */
+ * FP32 equation for the launches that follow.  MODE_ASM_INL: MODE_ASM with
+ * the face fluxes formed from u and v (valid while u and v are still the
+ * values the relaxation started from), and the sc correction left pending in
+ * ds32.  Nothing the assembly reads is written by the launch, so halo
+ * recomputation is race-free. */
 #define MODE_LOAD 0
 #define MODE_ASM 1
 #define MODE_ASM_STORE 2
+#define MODE_ASM_INL 3
+template <int S>
+struct RbShape {
+    static constexpr int TY = MGF_SWEEP_TILE_ROWS;
+    static constexpr int H = 2 * S, RW = BX + 2 * H, RH = TY + 2 * H, NR = RW * RH;
+};
+
+/* the work of one workgroup on tile T of level L; t: NR floats of LDS, mg
+ * and rows: the |div| rows of an assembling p' launch (TY x BX and TY
+ * doubles; unused otherwise) */
 template <int S, int NT, int MODE, int FIELD>
-__global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, int launch,
-                                              int last)
+__device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, int field_rt,
+                                           int launch, int last, float *t, double *mg,
+                                           double *rows)
 {
-    LEVEL(Ls);
     /* FIELD >= 0: the field is fixed at compile time (assembling launches);
      * -1: taken from field_rt */
     const int field = FIELD >= 0 ? FIELD : field_rt;
-    constexpr int TY = MGF_SWEEP_TILE_ROWS;
-    constexpr int H = 2 * S, RW = BX + 2 * H, RH = TY + 2 * H, NR = RW * RH;
+    constexpr int TY = RbShape<S>::TY;
+    constexpr int H = RbShape<S>::H, RW = RbShape<S>::RW, RH = RbShape<S>::RH;
+    constexpr int NR = RbShape<S>::NR;
     constexpr int NPT = (NR + NT - 1) / NT, NW = NT / BX;
     constexpr bool ASM = MODE != MODE_LOAD;
+    constexpr bool INL = MODE == MODE_ASM_INL;
     constexpr int MR = ASM ? TY : 1;
-    __shared__ float t[NR];
-    __shared__ double mg[MR][BX];
-    __shared__ double rows[MR];
     int nx, ny;
     extents(L, field, nx, ny);
     const gflt *src = launch == 0 ? nullptr : ((launch & 1) ? L.db32 : L.da32);
     gflt *dst = (launch & 1) ? L.da32 : L.db32;
-    /* the last launch of u / v leaves the correction pending in du32 / dv32 */
+    /* the last launch of u / v leaves the correction pending in du32 / dv32
+     * (and of sc in ds32 when the update applies it) */
     if (last && field == 0) dst = L.du32;
     if (last && field == 1) dst = L.dv32;
+    if (INL && field == 3) dst = L.ds32;
     const bool pfield = field == 2;
     const int npx = L.npx;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * TY - H;
     const int tid = threadIdx.y * BX + threadIdx.x;
     if (ASM && pfield) {
-        for (int k = tid; k < MR * BX; k += NT) mg[k / BX][k % BX] = 0.0;
+        for (int k = tid; k < MR * BX; k += NT) mg[k] = 0.0;
         __syncthreads();
     }
     Pt32 pt[NPT];
@@ -230,9 +260,9 @@ __global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, 
                     if (pfield) {
                         double mag;
                         pt[k] = asm_pt_p(L, gi, gj, mag);
-                        if (d >= H) mg[r - H][c - H] = mag;
+                        if (d >= H) mg[(r - H) * BX + (c - H)] = mag;
                     } else {
-                        pt[k] = asm_pt(L, field, gi, gj, apk[ASM ? k : 0]);
+                        pt[k] = asm_pt<INL>(L, field, gi, gj, apk[ASM ? k : 0]);
                     }
                 } else {
                     pt[k] = load_pt(L, g);
@@ -285,7 +315,7 @@ __global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, 
         const int w = threadIdx.y, lane = threadIdx.x;
 #pragma unroll
         for (int row = w; row < TY; row += NW) {
-            double v = mg[row][lane];
+            double v = mg[row * BX + lane];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
             if (lane == 0) rows[row] = v;
         }
@@ -299,6 +329,44 @@ __global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, 
             if (pb < L.nblk) L.partial[pb] = bs;
         }
     }
+}
+
+template <int S, int NT, int MODE, int FIELD>
+__global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, int launch,
+                                              int last)
+{
+    LEVEL(Ls);
+    constexpr int MR = MODE != MODE_LOAD ? RbShape<S>::TY : 1;
+    __shared__ float t[RbShape<S>::NR];
+    __shared__ double mg[MR * BX];
+    __shared__ double rows[MR];
+    rb32x_tile<S, NT, MODE, FIELD>(L, T, field_rt, launch, last, t, mg, rows);
+}
+
+/* The u, v and sc relaxations of a group in one launch (S sweeps each, the
+ * only launch of each): in mixed precision u and v keep their corrections
+ * pending until the update, so none of the three equations reads what
+ * another one changes, and the fluxes are formed from u and v.  The sc
+ * correction stays pending in ds32 until the update.  The grid is the sc
+ * tile grid with three logical z-slices per member (field = z % 3); tiles
+ * past u's or v's smaller range hold no unknowns and write nothing. */
+template <int S, int NT>
+__global__ void __launch_bounds__(NT) k_rb32x_uvs(const DevLevel *Ls)
+{
+    const TileId T3 = tile_id();
+    const TileId T{T3.x, T3.y, T3.z / 3};
+    const int f = T3.z - 3 * T.z;
+    const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[T.z];
+    if (L.ctl->done) return;
+    __shared__ float t[RbShape<S>::NR];
+    __shared__ double mg[BX];
+    __shared__ double rows[1];
+    if (f == 0)
+        rb32x_tile<S, NT, MODE_ASM_INL, 0>(L, T, 0, 0, 1, t, mg, rows);
+    else if (f == 1)
+        rb32x_tile<S, NT, MODE_ASM_INL, 1>(L, T, 1, 0, 1, t, mg, rows);
+    else
+        rb32x_tile<S, NT, MODE_ASM_INL, 3>(L, T, 3, 0, 1, t, mg, rows);
 }
 
 /* one FP32 Jacobi sweep of A d = r (wj = 1, V6:1241) */
@@ -348,12 +416,12 @@ __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int fi
 void mgf_launch_reduce_err(const DevGroup &G, int record, hipStream_t st);
 
 
-void mgf_launch_asm_mixed(const DevGroup &G, int field, int record, hipStream_t st)
+void mgf_launch_asm_mixed(const DevGroup &G, int field, int err, hipStream_t st)
 {
     if (field == 2) {
         hipLaunchKernelGGL(k_asm_p_mx, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st,
                            G.d);
-        mgf_launch_reduce_err(G, record, st);
+        if (err) mgf_launch_reduce_err(G, err == 2, st);
         return;
     }
     int ni = G.npx - 2, nj = G.npy - 2;
@@ -474,15 +542,16 @@ int mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, S
  * update of the FP64 unknown (field 0 u, 1 v, 2 p', 3 sc).  Red-black with
  * cfg.fuse: the first sweep launch assembles the equation itself, and for p'
  * the last launch writes pp; otherwise (Jacobi, no sweeps, or split
- * assembly) the assembly kernel stores the FP32 equation first.  The level's
- * sum |div| is reduced after the p' assembly (history entry when record). */
+ * assembly) the assembly kernel stores the FP32 equation first.  err: the
+ * level's sum |div| is reduced after the p' assembly (1), and is also the
+ * history entry of level-0 members (2), or is not needed (0). */
 void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int nsw,
-                                  SweepCfg cfg, int record, hipStream_t st)
+                                  SweepCfg cfg, int err, hipStream_t st)
 {
     int nx, ny;
     sweep_extents(G, field, nx, ny);
     if (isolve != 2 || nsw < 1 || !cfg.fuse || nx - 1 < 1 || ny - 1 < 1) {
-        mgf_launch_asm_mixed(G, field, record, st);
+        mgf_launch_asm_mixed(G, field, err, st);
         mgf_launch_smooth_mixed(G, field, isolve, nsw, cfg, st);
         return;
     }
@@ -500,11 +569,40 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
     }
     /* p': the last launch wrote pp; u, v: the last launch left the
      * correction pending in du32 / dv32; sc: apply it */
-    if (field == 2)
-        mgf_launch_reduce_err(G, record, st);
-    else if (field == 3)
+    if (field == 2) {
+        if (err) mgf_launch_reduce_err(G, err == 2, st);
+    } else if (field == 3) {
         hipLaunchKernelGGL(k_apply_mx, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, field, launch);
+    }
+}
+
+template <int S>
+static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
+{
+    const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_SWEEP_TILE_ROWS);
+    const dim3 g(g0.x, g0.y, 3 * g0.z);
+    if (nt == 1024)
+        hipLaunchKernelGGL((k_rb32x_uvs<S, 1024>), g, dim3(BX, 16), 0, st, G.d);
+    else if (nt == 512)
+        hipLaunchKernelGGL((k_rb32x_uvs<S, 512>), g, dim3(BX, 8), 0, st, G.d);
+    else
+        hipLaunchKernelGGL((k_rb32x_uvs<S, 256>), g, dim3(BX, 4), 0, st, G.d);
+}
+
+int mgf_launch_relax_uvs_mixed(const DevGroup &G, int isolve, int nsw, SweepCfg cfg,
+                               hipStream_t st)
+{
+    const int lim = cfg.spl < MGF_MAX_FUSED_SWEEPS ? cfg.spl : MGF_MAX_FUSED_SWEEPS;
+    if (isolve != 2 || !cfg.fuse || nsw < 1 || nsw > lim || G.any_outflow || G.npx - 3 < 1 ||
+        G.npy - 3 < 1)
+        return 0;
+    switch (nsw) {
+    case 1: launch_uvs_s<1>(G, cfg.nt_asm, st); break;
+    case 2: launch_uvs_s<2>(G, cfg.nt_asm, st); break;
+    default: launch_uvs_s<3>(G, cfg.nt_asm, st); break;
+    }
+    return 1;
 }
 
 /* the first launch of a fused red-black relaxation of a field on its own

@@ -48,6 +48,19 @@ struct Op {
     int g;                                /* group or pair index */
 };
 
+/* one V-cycle schedule over a set of instances: its launch groups and
+ * transfer pairs (descriptor arrays on the device) and, once captured, its
+ * graph */
+struct Program {
+    std::vector<int> members;             /* instance indices */
+    DevLevel *d_desc = nullptr;           /* all group descriptor arrays */
+    std::vector<DevGroup> groups;
+    std::vector<DevPair> pairs;
+    std::vector<Op> ops;                  /* one V-cycle */
+    hipGraphExec_t gexec = nullptr;
+    int graph_ready = 0;
+};
+
 struct mgf_solver {
     int nb = 0;
     std::vector<Inst> inst;
@@ -57,10 +70,8 @@ struct mgf_solver {
     int isolve = 0, nswpm = 0, nswpp = 0;
     void *arena = nullptr;
     size_t arena_bytes = 0;
-    DevLevel *d_desc = nullptr;           /* all group descriptor arrays */
-    std::vector<DevGroup> groups;
-    std::vector<DevPair> pairs;
-    std::vector<Op> ops;                  /* one V-cycle */
+    Program full;                         /* every instance */
+    Program live;                         /* the instances still running (compaction) */
     /* single-instance step API: group of level ig, pair with fine level ig */
     std::vector<int> level_group;
     std::vector<int> level_pair;
@@ -68,8 +79,6 @@ struct mgf_solver {
     DevCtl *h_ctl = nullptr;              /* pinned mirror */
     double *d_hist = nullptr;             /* nb x hist_cap */
     int hist_cap = 0;
-    hipGraphExec_t gexec = nullptr;
-    int graph_ready = 0;
     char err[512] = {0};
 };
 
@@ -114,7 +123,7 @@ static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
     size_t t = 27 * align256(n * sizeof(real_t));  /* 23 arrays + 4 initial-state copies */
-    if (mixed) t += 10 * align256(n * sizeof(float));
+    if (mixed) t += 11 * align256(n * sizeof(float));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
     t += align256(n * sizeof(uint64_t));            /* sccode */
@@ -237,9 +246,9 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
         L.partial = (double *)take((size_t)L.nblk * sizeof(double));
         L.isolve = c->isolve;
         if (s->opt.precision == MGF_PREC_MIXED) {
-            float **a32[10] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
-                               &L.da32, &L.db32, &L.du32, &L.dv32};
-            for (int k = 0; k < 10; ++k) *a32[k] = (float *)take(n * sizeof(float));
+            float **a32[11] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
+                               &L.da32, &L.db32, &L.du32, &L.dv32, &L.ds32};
+            for (int k = 0; k < 11; ++k) *a32[k] = (float *)take(n * sizeof(float));
         }
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
@@ -291,7 +300,9 @@ static void fill_group_meta(DevGroup &G, const std::vector<DevLevel> &m)
     G.npy = m[0].npy;
     G.max_npcopy = 0;
     G.any_pbound_x = G.any_pbound_y = G.any_outflow = 0;
+    G.any_finest = 0;
     for (const DevLevel &L : m) {
+        if (L.ig == 0) G.any_finest = 1;
         if (L.npcopy > G.max_npcopy) G.max_npcopy = L.npcopy;
         if (L.P.ibc[0] == 3) G.any_pbound_x = 1;
         if (L.P.ibc[2] == 3) G.any_pbound_y = 1;
@@ -304,10 +315,10 @@ static void fill_group_meta(DevGroup &G, const std::vector<DevLevel> &m)
  * relaxed on the way down; kind 1: those that are not their instance's
  * finest level, relaxed again on the way up).  Pairs: the same-shape levels
  * at depth d >= 1 with their next coarser levels, for the transfers. */
-static int build_program(mgf_solver *s)
+static int build_program(mgf_solver *s, Program &P, const std::vector<int> &members)
 {
     int D = 0;
-    for (const Inst &I : s->inst) D = I.ng - 1 > D ? I.ng - 1 : D;
+    for (int b : members) D = s->inst[b].ng - 1 > D ? s->inst[b].ng - 1 : D;
     struct Key {
         int d, nx, ny, kind;
         bool operator==(const Key &o) const {
@@ -324,7 +335,7 @@ static int build_program(mgf_solver *s)
     };
     std::vector<std::vector<int>> down_g(D + 1), up_g(D + 1), pr(D + 1);
     for (int d = D; d >= 0; --d) {
-        for (int b = 0; b < s->nb; ++b) {
+        for (int b : members) {
             const Inst &I = s->inst[b];
             const int ig = I.ng - 1 - d;
             if (ig < 0) continue;
@@ -369,23 +380,24 @@ static int build_program(mgf_solver *s)
     size_t ndesc = 0;
     for (auto &g : gm) ndesc += g.size();
     for (auto &p : pf) ndesc += 2 * p.size();
-    HIPCHK(s, hipMalloc(&s->d_desc, ndesc * sizeof(DevLevel)));
+    P.members = members;
+    HIPCHK(s, hipMalloc(&P.d_desc, ndesc * sizeof(DevLevel)));
     std::vector<DevLevel> host;
     host.reserve(ndesc);
-    s->groups.assign(gm.size(), DevGroup());
-    s->pairs.assign(pf.size(), DevPair());
+    P.groups.assign(gm.size(), DevGroup());
+    P.pairs.assign(pf.size(), DevPair());
     for (const auto &e : gkey) {
-        s->groups[e.second].depth = e.first.d;
-        s->groups[e.second].up = e.first.kind;
+        P.groups[e.second].depth = e.first.d;
+        P.groups[e.second].up = e.first.kind;
     }
     for (size_t g = 0; g < gm.size(); ++g) {
-        DevGroup &G = s->groups[g];
-        G.d = s->d_desc + host.size();
+        DevGroup &G = P.groups[g];
+        G.d = P.d_desc + host.size();
         fill_group_meta(G, gm[g]);
         host.insert(host.end(), gm[g].begin(), gm[g].end());
     }
     for (size_t p = 0; p < pf.size(); ++p) {
-        DevPair &R = s->pairs[p];
+        DevPair &R = P.pairs[p];
         R.count = (int)pf[p].size();
         R.npxf = pf[p][0].npx;
         R.npyf = pf[p][0].npy;
@@ -394,15 +406,15 @@ static int build_program(mgf_solver *s)
         for (const DevLevel &c : pc[p])
             if (c.npx != R.npxc || c.npy != R.npyc)
                 return set_err(s, MGF_ERR_GEOMETRY, "coarse shapes differ within a transfer group");
-        R.f = s->d_desc + host.size();
+        R.f = P.d_desc + host.size();
         host.insert(host.end(), pf[p].begin(), pf[p].end());
-        R.c = s->d_desc + host.size();
+        R.c = P.d_desc + host.size();
         host.insert(host.end(), pc[p].begin(), pc[p].end());
     }
-    HIPCHK(s, hipMemcpy(s->d_desc, host.data(), host.size() * sizeof(DevLevel),
+    HIPCHK(s, hipMemcpy(P.d_desc, host.data(), host.size() * sizeof(DevLevel),
                         hipMemcpyHostToDevice));
-    s->ops = ops;
-    if (s->nb == 1) {
+    P.ops = ops;
+    if (&P == &s->full && s->nb == 1) {
         const Inst &I = s->inst[0];
         s->level_group.assign(I.ng, -1);
         s->level_pair.assign(I.ng, -1);
@@ -539,7 +551,11 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
         for (int b = 0; b < n && !rc; ++b)
             rc = upload_instance(s, s->inst[b], cur, s->d_ctl + b, seeds ? &seeds[b] : NULL);
     }
-    if (!rc) rc = build_program(s);
+    if (!rc) {
+        std::vector<int> all(n);
+        for (int b = 0; b < n; ++b) all[b] = b;
+        rc = build_program(s, s->full, all);
+    }
     if (!rc) {
         memset(s->h_ctl, 0, (size_t)n * sizeof(DevCtl));
         for (int b = 0; b < n; ++b) s->h_ctl[b].hist = s->d_hist + (size_t)b * s->hist_cap;
@@ -564,13 +580,21 @@ extern "C" int mgf_create(const mgf_problem *p, const mgf_options *opt, mgf_solv
     return mgf_batch_create(one, 1, opt, out);
 }
 
+/* the caller has synchronised the stream the program's graph ran on */
+static void release_program(Program &P)
+{
+    if (P.graph_ready) (void)hipGraphExecDestroy(P.gexec);
+    if (P.d_desc) (void)hipFree(P.d_desc);
+    P = Program();
+}
+
 extern "C" void mgf_destroy(mgf_solver *s)
 {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
-    if (s->graph_ready) (void)hipGraphExecDestroy(s->gexec);
+    release_program(s->live);
+    release_program(s->full);
     if (s->arena) (void)hipFree(s->arena);
-    if (s->d_desc) (void)hipFree(s->d_desc);
     if (s->d_hist) (void)hipFree(s->d_hist);
     if (s->d_ctl) (void)hipFree(s->d_ctl);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
@@ -607,18 +631,34 @@ extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
 /* schedule                                                                 */
 /* ---------------------------------------------------------------------- */
 
-/* V6:201-207 (leg 0) and V6:237-243 (leg 1) for a group */
+/* V6:201-207 (leg 0) and V6:237-243 (leg 1) for a group.  err: whether the
+ * level's sum |div| is reduced (1) and recorded as the history entry of its
+ * level-0 members (2); masserror computes it on every call, but only the
+ * value of the finest level is recorded and read (V6:716-723, 248), so a
+ * V-cycle reduces it where it is recorded and the step call where it
+ * returns it. */
 /* mixed precision: the same steps with FP64 assembly + residual, FP32
  * correction sweeps and FP64 apply (mgf_kernels_mixed.hip) */
-static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg)
+static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int err)
 {
     hipStream_t st = s->st;
     const SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads,
                                        s->opt.split_assembly);
+    /* u, v and sc in one launch where the configuration allows: sc's
+     * equation reads only the fluxes (those of the u and v the relaxation
+     * starts from), sc itself and its penalty / tau terms, none of which
+     * the update or the periodic copies write; its correction is applied by
+     * the update */
+    if (mgf_launch_relax_uvs_mixed(G, s->isolve, s->nswpm, cfg, st)) {
+        mgf_launch_relax_field_mixed(G, 2, s->isolve, s->nswpp, cfg, err, st);
+        mgf_launch_update(G, 2, st);
+        mgf_launch_pbound(G, st);
+        return;
+    }
     mgf_launch_fluxes(G, st);
     mgf_launch_relax_field_mixed(G, 0, s->isolve, s->nswpm, cfg, 0, st);
     mgf_launch_relax_field_mixed(G, 1, s->isolve, s->nswpm, cfg, 0, st);
-    mgf_launch_relax_field_mixed(G, 2, s->isolve, s->nswpp, cfg, leg == 0, st);
+    mgf_launch_relax_field_mixed(G, 2, s->isolve, s->nswpp, cfg, err, st);
     mgf_launch_update(G, 1, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) {
@@ -630,10 +670,10 @@ static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg)
     }
 }
 
-static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg)
+static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err)
 {
     if (s->opt.precision == MGF_PREC_MIXED) {
-        enqueue_relax_mixed(s, G, leg);
+        enqueue_relax_mixed(s, G, leg, err);
         return;
     }
     hipStream_t st = s->st;
@@ -642,7 +682,7 @@ static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg)
     mgf_launch_smooth(G, 0, s->isolve, s->nswpm, st);
     mgf_launch_asm_v(G, st);
     mgf_launch_smooth(G, 1, s->isolve, s->nswpm, st);
-    mgf_launch_asm_p(G, leg == 0, st);
+    mgf_launch_asm_p(G, err, st);
     mgf_launch_smooth(G, 2, s->isolve, s->nswpp, st);
     mgf_launch_update(G, 0, st);
     mgf_launch_pbound(G, st);
@@ -657,14 +697,16 @@ static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg)
     }
 }
 
-static void enqueue_vcycle(mgf_solver *s)
+static void enqueue_vcycle(mgf_solver *s, const Program &P)
 {
-    for (const Op &o : s->ops) {
+    for (const Op &o : P.ops) {
         switch (o.kind) {
-        case OP_RELAX_DOWN: enqueue_relax(s, s->groups[o.g], 0); break;
-        case OP_RELAX_UP: enqueue_relax(s, s->groups[o.g], 1); break;
-        case OP_RESTRICT: mgf_launch_restrict(s->pairs[o.g], s->st); break;
-        case OP_PROLONG: mgf_launch_prolong(s->pairs[o.g], s->st); break;
+        case OP_RELAX_DOWN:
+            enqueue_relax(s, P.groups[o.g], 0, P.groups[o.g].any_finest ? 2 : 0);
+            break;
+        case OP_RELAX_UP: enqueue_relax(s, P.groups[o.g], 1, 0); break;
+        case OP_RESTRICT: mgf_launch_restrict(P.pairs[o.g], s->st); break;
+        case OP_PROLONG: mgf_launch_prolong(P.pairs[o.g], s->st); break;
         }
     }
 }
@@ -744,7 +786,7 @@ extern "C" int mgf_relax(mgf_solver *s, int ig, int leg, double *mass_err)
         return set_err(s, MGF_ERR_ARG, "bad level or leg");
     if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
     /* a leg-0 relaxation of level 0 also writes history slot 0 (unused here) */
-    enqueue_relax(s, s->groups[s->level_group[ig]], leg);
+    enqueue_relax(s, s->full.groups[s->level_group[ig]], leg, leg == 0 ? 2 : 1);
     if ((rc = check_launch(s))) return rc;
     if ((rc = read_ctl(s))) return rc;
     if (mass_err) *mass_err = s->h_ctl[0].err[ig];
@@ -758,7 +800,7 @@ extern "C" int mgf_restrict(mgf_solver *s, int ig)
     if (rc) return rc;
     if (ig < 0 || ig + 1 >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad fine level");
     if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
-    mgf_launch_restrict(s->pairs[s->level_pair[ig]], s->st);
+    mgf_launch_restrict(s->full.pairs[s->level_pair[ig]], s->st);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
@@ -771,7 +813,7 @@ extern "C" int mgf_prolong(mgf_solver *s, int igc)
     if (rc) return rc;
     if (igc < 1 || igc >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad coarse level");
     if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
-    mgf_launch_prolong(s->pairs[s->level_pair[igc - 1]], s->st);
+    mgf_launch_prolong(s->full.pairs[s->level_pair[igc - 1]], s->st);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
@@ -784,7 +826,7 @@ extern "C" int mgf_vcycle(mgf_solver *s, int icyc, double *mass_err)
     if (rc) return rc;
     if ((rc = ensure_hist(s, icyc + 1))) return rc;
     if ((rc = reset_ctl(s, icyc, NULL, NULL))) return rc;
-    enqueue_vcycle(s);
+    enqueue_vcycle(s, s->full);
     if ((rc = check_launch(s))) return rc;
     double h = 0.0;
     HIPCHK(s, hipMemcpyAsync(&h, s->d_hist + icyc, sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -795,21 +837,21 @@ extern "C" int mgf_vcycle(mgf_solver *s, int icyc, double *mass_err)
 
 /* the graph holds no history or control pointers of its own: those live in
  * the control blocks, so it stays valid when the history buffer grows */
-static int capture_graph(mgf_solver *s)
+static int capture_graph(mgf_solver *s, Program &P)
 {
-    if (s->graph_ready) return MGF_OK;
+    if (P.graph_ready) return MGF_OK;
     hipGraph_t g;
     HIPCHK(s, hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-    enqueue_vcycle(s);
+    enqueue_vcycle(s, P);
     mgf_launch_end_cycle(s->d_ctl, s->nb, s->st);
     hipError_t e = hipStreamEndCapture(s->st, &g);
     if (e != hipSuccess)
         return set_err(s, MGF_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&s->gexec, g, NULL, NULL, 0);
+    e = hipGraphInstantiate(&P.gexec, g, NULL, NULL, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess)
         return set_err(s, MGF_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
-    s->graph_ready = 1;
+    P.graph_ready = 1;
     return MGF_OK;
 }
 
@@ -817,7 +859,15 @@ static int capture_graph(mgf_solver *s)
  * below its tolerance, or it ran its cycle limit (both decided on the
  * device).  The host enqueues cycles in growing chunks and polls the control
  * blocks between chunks; kernels of finished instances return at once, so
- * cycles enqueued past an instance's end change nothing. */
+ * cycles enqueued past an instance's end change nothing.
+ *
+ * Compaction: launches over stopped instances still cost their dispatch
+ * (and a group whose members have all stopped, its launch latency), which
+ * in a batch whose slowest members run far longer than the rest is a
+ * sizeable share of the tail.  When at most 3/4 of the instances of the
+ * running schedule are still live, a schedule over the live ones alone is
+ * built and captured, and replaces it from the next chunk on.  Every
+ * instance's arithmetic is its own, so the results do not change. */
 static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
 {
     int maxnv = 0;
@@ -825,24 +875,38 @@ static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
     int rc = ensure_hist(s, maxnv > 0 ? maxnv : 1);
     if (rc) return rc;
     if ((rc = reset_ctl(s, 0, tol, limit))) return rc;
-    if (s->opt.use_graph && (rc = capture_graph(s))) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->st));
+    release_program(s->live);
+    Program *P = &s->full;
+    if (s->opt.use_graph && (rc = capture_graph(s, *P))) return rc;
     int launched = 0, chunk = 1;
     while (launched < maxnv) {
         const int k = chunk < maxnv - launched ? chunk : maxnv - launched;
         for (int t = 0; t < k; ++t) {
             if (s->opt.use_graph) {
-                HIPCHK(s, hipGraphLaunch(s->gexec, s->st));
+                HIPCHK(s, hipGraphLaunch(P->gexec, s->st));
             } else {
-                enqueue_vcycle(s);
+                enqueue_vcycle(s, *P);
                 mgf_launch_end_cycle(s->d_ctl, s->nb, s->st);
             }
         }
         if ((rc = check_launch(s))) return rc;
         launched += k;
-        if ((rc = read_ctl(s))) return rc;
-        int all = 1;
-        for (int b = 0; b < s->nb; ++b) all &= s->h_ctl[b].done;
-        if (all) break;
+        if ((rc = read_ctl(s))) return rc;       /* the stream is idle from here */
+        std::vector<int> alive;
+        for (int b : P->members)
+            if (!s->h_ctl[b].done) alive.push_back(b);
+        if (alive.empty()) break;
+        if (s->opt.use_graph && s->opt.compact && 4 * alive.size() <= 3 * P->members.size()) {
+            Program np;
+            if ((rc = build_program(s, np, alive)) || (rc = capture_graph(s, np))) {
+                release_program(np);
+                return rc;
+            }
+            release_program(s->live);
+            s->live = np;
+            P = &s->live;
+        }
         if (chunk < 32) chunk *= 2;
     }
     return read_ctl(s);
@@ -1023,12 +1087,12 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     const int d = s->inst[0].ng - 1 - ig;
     const DevLevel &L0 = s->inst[0].lv[ig];
     int g = -1;
-    for (size_t k = 0; k < s->groups.size(); ++k) {
-        const DevGroup &G = s->groups[k];
+    for (size_t k = 0; k < s->full.groups.size(); ++k) {
+        const DevGroup &G = s->full.groups[k];
         if (!G.up && G.depth == d && G.npx == L0.npx && G.npy == L0.npy) g = (int)k;
     }
     if (g < 0) return set_err(s, MGF_ERR_ARG, "no group for level %d", ig);
-    const DevGroup &G = s->groups[g];
+    const DevGroup &G = s->full.groups[g];
     const int field = (kernel == MGF_KERNEL_SWEEP_PP || kernel == MGF_KERNEL_RELAX_PP) ? 2 : 0;
     const int isolve = s->isolve == 1 ? 1 : 2;
     const int even = reps + (reps & 1);    /* an even count ends without a copy-back */

@@ -111,6 +111,12 @@ void mgf_launch_relax_field_mixed(const DevGroup &, int, int, int, SweepCfg, int
     not_here("mgf_launch_relax_field_mixed");
 }
 
+int mgf_launch_relax_uvs_mixed(const DevGroup &, int, int, SweepCfg, hipStream_t)
+{
+    not_here("mgf_launch_relax_uvs_mixed");
+    return 0;
+}
+
 int mgf_launch_sweeps_mixed(const DevGroup &, int, int, int, SweepCfg, hipStream_t)
 {
     not_here("mgf_launch_sweeps_mixed");

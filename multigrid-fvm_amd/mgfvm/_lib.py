@@ -45,7 +45,7 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("device", ctypes.c_int), ("strict_limits", ctypes.c_int),
                 ("sweeps_per_launch", ctypes.c_int), ("sweep_threads", ctypes.c_int),
-                ("split_assembly", ctypes.c_int)]
+                ("split_assembly", ctypes.c_int), ("compact", ctypes.c_int)]
 
 
 class ProblemStruct(ctypes.Structure):
@@ -281,7 +281,7 @@ class Batch:
 
     def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, sweeps_per_launch: int = 0,
-                 sweep_threads: int = 0, split_assembly: bool = False):
+                 sweep_threads: int = 0, split_assembly: bool = False, compact: bool = True):
         L = lib()
         probs = [read_problem(a, b) for a, b in pairs]
         try:
@@ -294,6 +294,7 @@ class Batch:
             opt.sweeps_per_launch = sweeps_per_launch
             opt.sweep_threads = sweep_threads
             opt.split_assembly = 1 if split_assembly else 0
+            opt.compact = 1 if compact else 0
             h = ctypes.c_void_p()
             rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))
             if rc != 0:

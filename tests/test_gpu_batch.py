@@ -165,3 +165,36 @@ def test_sweep_launch_shapes_bit_identical(tmp_path, spl, threads, split):
                                   bt.get(k, 0, name).view(np.uint64)), (k, name)
     base.close()
     bt.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_compaction_result_neutral(tmp_path, precision):
+    """Re-capturing the V-cycle over the still-running members (members stop
+    at cycles 2 .. 90, so the schedule is rebuilt several times, down to one
+    member) gives every member the history and fields of a batch that keeps
+    launching over all of them; a second solve after a reset rebuilds the
+    schedules again and repeats the first bit for bit."""
+    import mgfvm
+    prec = mgfvm.PREC_FP64 if precision == "fp64" else mgfvm.PREC_MIXED
+    base = _mixed_problems()
+    limits = [2, 5, 9, 17, 33, 60, 90]
+    probs = [p.replace(nvcycle=lim, convergence=1e-14) for p, lim in zip(base, limits)]
+    pairs = [p.write(str(tmp_path / f"c{k}")) for k, p in enumerate(probs)]
+    _say(f"[gpu] compaction {precision}")
+    ref = mgfvm.Batch(pairs, precision=prec, compact=False)
+    nr, cr = ref.solve()
+    assert list(nr) == limits
+    bt = mgfvm.Batch(pairs, precision=prec)
+    for rep in range(2):
+        if rep:
+            bt.reset()
+        nc, cv = bt.solve()
+        assert (list(nc), list(cv)) == (list(nr), list(cr)), rep
+        for k in range(len(probs)):
+            assert np.array_equal(ref.history(k), bt.history(k)), (rep, k)
+            for ig in range(probs[k].ngrid):
+                for name in STATE if prec == mgfvm.PREC_FP64 else ("u", "v", "p", "sc"):
+                    assert np.array_equal(ref.get(k, ig, name).view(np.uint64),
+                                          bt.get(k, ig, name).view(np.uint64)), (rep, k, ig, name)
+    ref.close()
+    bt.close()
