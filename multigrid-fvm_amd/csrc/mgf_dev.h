@@ -152,7 +152,8 @@ void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
 /* field: 0 u, 1 v, 2 pp, 3 sc */
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1),
- * and the sc correction in ds32 as well (2) */
+ * and the sc correction in ds32 as well, with the updated p of the copy
+ * origins already in tmp (2) */
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);
@@ -169,7 +170,9 @@ void mgf_launch_asm_mixed(const DevGroup &G, int field, int err, hipStream_t st)
 struct SweepCfg {
     int spl, nt, nt_asm, fuse;
 };
+#ifndef MGF_SWEEP_TILE_ROWS
 #define MGF_SWEEP_TILE_ROWS 16
+#endif
 #define MGF_AUTO_SWEEPS_PER_LAUNCH 3
 #define MGF_MAX_SWEEPS_PER_LAUNCH 6
 /* sweeps of a launch that also assembles the equation */
@@ -184,13 +187,15 @@ double mgf_launch_relax_first_mixed(const DevGroup &G, int field, int nsw, Sweep
 /* assembly + sweeps + FP64 update of one field in a relaxation */
 void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int nsw,
                                   SweepCfg cfg, int err, hipStream_t st);
-/* u, v and sc of a relaxation in one launch, sc's correction left pending
- * for mgf_launch_update(G, 2); 0 (nothing launched) when the group or the
- * configuration does not allow it: red-black with fused assembly and nsw
- * within one launch, and no outflow boundary (which rewrites sc's ghost
- * between the update and the sc solve) */
-int mgf_launch_relax_uvs_mixed(const DevGroup &G, int isolve, int nsw, SweepCfg cfg,
-                               hipStream_t st);
+/* A mixed relaxation up to its update with u, v and sc in one launch: that
+ * launch, p' (whose last launch also stashes the updated p of the copy
+ * origins), the sum |div| as `err` says, and the update, which applies sc's
+ * pending correction too.  0 (nothing launched) when the group or the
+ * configuration does not allow it: red-black with fused assembly, nswm
+ * within one launch, nswp >= 1, and no outflow boundary (which rewrites
+ * sc's ghost between the update and the sc solve). */
+int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
+                                  SweepCfg cfg, int err, hipStream_t st);
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
                              hipStream_t st);
 /* the FP32 sweeps alone (they only touch the correction buffers); returns

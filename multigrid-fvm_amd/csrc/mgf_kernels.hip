@@ -341,7 +341,9 @@ __global__ void k_pstash(const DevLevel *Ls)
  * solves are still in du32 / dv32 over their swept ranges (the value of u
  * there is u + du, formed exactly as the solve's own update would); 2: the
  * sc solve ran in the same launch as u and v, and its correction in ds32 is
- * applied here (sc += d, as k_apply_mx would; the update does not read sc) */
+ * applied here (sc += d, as k_apply_mx would; the update does not read sc),
+ * and the last p' launch left the updated p of every copy origin in tmp
+ * (the value k_pstash would stash) */
 template <int PENDING>
 __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
 {
@@ -360,10 +362,20 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
     L.u[ij] = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
     L.v[ij] = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
     if (PENDING == 2) L.sc[ij] = L.sc[ij] + (double)L.ds32[ij];
-    if (L.flags[ij] & MGF_F_PCOPY)
-        L.p[ij] = L.pstash[L.pslot[ij]];
-    else
+    if (L.flags[ij] & MGF_F_PCOPY) {
+        if (PENDING == 2) {
+            /* the origin's updated p, stashed by the last p' launch; an
+             * origin outside the updated range keeps its p */
+            const int o = L.porig[L.pslot[ij]];
+            const int oi = o % nx, oj = o / nx;
+            const bool in = (oi >= 1 && oi <= nx - 2 && oj >= 1 && oj <= L.npy - 2);
+            L.p[ij] = in ? L.tmp[o] : L.p[o];
+        } else {
+            L.p[ij] = L.pstash[L.pslot[ij]];
+        }
+    } else {
         L.p[ij] = L.p[ij] + P.relxp * (pp - 0.0);
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -798,7 +810,7 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
 
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st)
 {
-    if (G.max_npcopy > 0)
+    if (G.max_npcopy > 0 && pending != 2)
         hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);

@@ -229,6 +229,7 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     if (last && field == 1) dst = L.dv32;
     if (INL && field == 3) dst = L.ds32;
     const bool pfield = field == 2;
+    const bool stash = pfield && (last & 2) != 0;
     const int npx = L.npx;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * TY - H;
     const int tid = threadIdx.y * BX + threadIdx.x;
@@ -241,7 +242,8 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
      * no global store between the assembly loads) */
     double apk[ASM ? NPT : 1];
     /* per slot: distance to the region's edge (bits 0-7), colour (bit 8),
-     * inside the swept range (bit 9) */
+     * inside the swept range (bit 9), a copy origin whose updated p is
+     * stashed (bit 10) */
     unsigned meta[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
@@ -269,6 +271,7 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                 }
                 if (src) val = src[g];
                 m = (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
+                if (stash && d >= H && (L.flags[g] & MGF_F_PORIG)) m |= 1u << 10;
             }
             t[p] = val;
         }
@@ -299,10 +302,16 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
             const int p = tid + k * NT;
             const int r = p / RW, c = p - r * RW;
             const int g = (rj0 + r) * npx + (ri0 + c);
-            if (to_pp)
-                L.pp[g] = (double)t[p];
-            else
+            if (to_pp) {
+                const double ppv = (double)t[p];
+                L.pp[g] = ppv;
+                /* the freshly updated p of a copy origin, for the update's
+                 * obstacle-edge copies (what k_pstash would stash) */
+                if (m & (1u << 10))
+                    L.tmp[g] = L.p[g] + L.P.relxp * (ppv - 0.0);
+            } else {
                 dst[g] = t[p];
+            }
             if (ASM && field == 0) L.apu[g] = apk[ASM ? k : 0];
             if (ASM && field == 1) L.apv[g] = apk[ASM ? k : 0];
             if (MODE == MODE_ASM_STORE) store_pt(L, g, pt[k]);
@@ -538,6 +547,64 @@ int mgf_launch_sweeps_mixed(const DevGroup &G, int field, int isolve, int nsw, S
     return launch;
 }
 
+/* A level whose interior fits in one tile column and four tile rows (the
+ * coarse levels of a hierarchy): its launches are latency-bound, so they
+ * take 1024-thread workgroups and all sweeps of a relaxation in the one
+ * assembling launch (its halo then spans most of the level anyway). */
+static bool small_group(const DevGroup &G)
+{
+    return G.npx - 2 <= BX && G.npy - 2 <= 4 * MGF_SWEEP_TILE_ROWS;
+}
+
+template <int S>
+static void launch_small_s(const DevGroup &G, int field, int last, int nx, int ny,
+                           hipStream_t st)
+{
+    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SWEEP_TILE_ROWS);
+    const dim3 b(BX, 16);
+    switch (field) {
+    case 0: hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM, 0>), g, b, 0, st, G.d, 0, 0, last); break;
+    case 1: hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM, 1>), g, b, 0, st, G.d, 1, 0, last); break;
+    case 2: hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM, 2>), g, b, 0, st, G.d, 2, 0, last); break;
+    default: hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM, 3>), g, b, 0, st, G.d, 3, 0, last); break;
+    }
+}
+
+/* the launches of a fused red-black relaxation of one field (the first one
+ * assembles); lastbits: the `last` argument of the final launch (bit 0:
+ * final; bit 1, p' only: also stash the updated p of copy origins in tmp).
+ * Returns the number of launches. */
+static int launch_fused(const DevGroup &G, int field, int nsw, SweepCfg cfg, int lastbits,
+                        hipStream_t st)
+{
+    int nx, ny;
+    sweep_extents(G, field, nx, ny);
+    if (small_group(G) && nsw <= MGF_MAX_SWEEPS_PER_LAUNCH) {
+        switch (nsw) {
+        case 1: launch_small_s<1>(G, field, lastbits, nx, ny, st); break;
+        case 2: launch_small_s<2>(G, field, lastbits, nx, ny, st); break;
+        case 3: launch_small_s<3>(G, field, lastbits, nx, ny, st); break;
+        case 4: launch_small_s<4>(G, field, lastbits, nx, ny, st); break;
+        case 5: launch_small_s<5>(G, field, lastbits, nx, ny, st); break;
+        default: launch_small_s<6>(G, field, lastbits, nx, ny, st); break;
+        }
+        return 1;
+    }
+    int launch = 0;
+    for (int done = 0; done < nsw; ++launch) {
+        int lim = cfg.spl;
+        if (launch == 0 && lim > MGF_MAX_FUSED_SWEEPS) lim = MGF_MAX_FUSED_SWEEPS;
+        const int n = nsw - done < lim ? nsw - done : lim;
+        const int last = done + n == nsw ? lastbits : 0;
+        if (launch > 0)
+            launch_load(G, field, launch, last, n, nx, ny, cfg.nt, st);
+        else
+            launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
+        done += n;
+    }
+    return launch;
+}
+
 /* smoothing of one field in a relaxation: assembly, nsw sweeps and the
  * update of the FP64 unknown (field 0 u, 1 v, 2 p', 3 sc).  Red-black with
  * cfg.fuse: the first sweep launch assembles the equation itself, and for p'
@@ -555,18 +622,7 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
         mgf_launch_smooth_mixed(G, field, isolve, nsw, cfg, st);
         return;
     }
-    int launch = 0;
-    for (int done = 0; done < nsw; ++launch) {
-        int lim = cfg.spl;
-        if (launch == 0 && lim > MGF_MAX_FUSED_SWEEPS) lim = MGF_MAX_FUSED_SWEEPS;
-        const int n = nsw - done < lim ? nsw - done : lim;
-        const int last = done + n == nsw;
-        if (launch > 0)
-            launch_load(G, field, launch, last, n, nx, ny, cfg.nt, st);
-        else
-            launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
-        done += n;
-    }
+    const int launch = launch_fused(G, field, nsw, cfg, 1, st);
     /* p': the last launch wrote pp; u, v: the last launch left the
      * correction pending in du32 / dv32; sc: apply it */
     if (field == 2) {
@@ -590,18 +646,21 @@ static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
         hipLaunchKernelGGL((k_rb32x_uvs<S, 256>), g, dim3(BX, 4), 0, st, G.d);
 }
 
-int mgf_launch_relax_uvs_mixed(const DevGroup &G, int isolve, int nsw, SweepCfg cfg,
-                               hipStream_t st)
+int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
+                                  SweepCfg cfg, int err, hipStream_t st)
 {
     const int lim = cfg.spl < MGF_MAX_FUSED_SWEEPS ? cfg.spl : MGF_MAX_FUSED_SWEEPS;
-    if (isolve != 2 || !cfg.fuse || nsw < 1 || nsw > lim || G.any_outflow || G.npx - 3 < 1 ||
-        G.npy - 3 < 1)
+    if (isolve != 2 || !cfg.fuse || nswm < 1 || nswm > lim || nswp < 1 || G.any_outflow ||
+        G.npx - 3 < 1 || G.npy - 3 < 1)
         return 0;
-    switch (nsw) {
+    switch (nswm) {
     case 1: launch_uvs_s<1>(G, cfg.nt_asm, st); break;
     case 2: launch_uvs_s<2>(G, cfg.nt_asm, st); break;
     default: launch_uvs_s<3>(G, cfg.nt_asm, st); break;
     }
+    (void)launch_fused(G, 2, nswp, cfg, 1 | 2, st);
+    if (err) mgf_launch_reduce_err(G, err == 2, st);
+    mgf_launch_update(G, 2, st);
     return 1;
 }
 

@@ -394,6 +394,7 @@ int mgf_plan_build(const mgf_problem *p, int strict, mgf_plan **out, char *err, 
                 l->pslot[k] = cnt;
                 l->porig[cnt++] = src[k];
                 l->flags[k] |= MGF_F_PCOPY;
+                l->flags[src[k]] |= MGF_F_PORIG;
             }
         free(src);
     }

@@ -34,6 +34,7 @@ extern "C" {
 #define MGF_F_OBS    0x08u   /* obstacle cell: tag = 0 */
 #define MGF_F_SCADD  0x10u   /* sccode[ij] holds obstacle diffusion adds */
 #define MGF_F_PCOPY  0x20u   /* p[ij] is overwritten by an obstacle-edge copy */
+#define MGF_F_PORIG  0x40u   /* some obstacle-edge copy ends with this cell's updated p */
 
 /* sccode layout: counts of x-type adds to ae (bits 0-4) and aw (5-9), of
  * y-type adds to an (10-14) and as (15-19), number of adds to ap (20-25),

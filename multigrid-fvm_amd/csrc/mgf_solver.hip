@@ -649,9 +649,7 @@ static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int e
      * starts from), sc itself and its penalty / tau terms, none of which
      * the update or the periodic copies write; its correction is applied by
      * the update */
-    if (mgf_launch_relax_uvs_mixed(G, s->isolve, s->nswpm, cfg, st)) {
-        mgf_launch_relax_field_mixed(G, 2, s->isolve, s->nswpp, cfg, err, st);
-        mgf_launch_update(G, 2, st);
+    if (mgf_launch_relax_merged_mixed(G, s->isolve, s->nswpm, s->nswpp, cfg, err, st)) {
         mgf_launch_pbound(G, st);
         return;
     }

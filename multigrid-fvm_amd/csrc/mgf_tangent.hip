@@ -111,9 +111,9 @@ void mgf_launch_relax_field_mixed(const DevGroup &, int, int, int, SweepCfg, int
     not_here("mgf_launch_relax_field_mixed");
 }
 
-int mgf_launch_relax_uvs_mixed(const DevGroup &, int, int, SweepCfg, hipStream_t)
+int mgf_launch_relax_merged_mixed(const DevGroup &, int, int, int, SweepCfg, int, hipStream_t)
 {
-    not_here("mgf_launch_relax_uvs_mixed");
+    not_here("mgf_launch_relax_merged_mixed");
     return 0;
 }
 
