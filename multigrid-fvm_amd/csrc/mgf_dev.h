@@ -157,7 +157,11 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);
-void mgf_launch_restrict(const DevPair &R, hipStream_t st);
+/* restv (unless an update_restv launch did it) and the tau terms */
+void mgf_launch_restrict(const DevPair &R, int restv, hipStream_t st);
+/* the update of a down-leg group fused with restv onto its coarser levels
+ * (R: the transfer pair of the same levels, members in the same order) */
+void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, hipStream_t st);
 void mgf_launch_prolong(const DevPair &R, hipStream_t st);
 void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
@@ -193,9 +197,11 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
  * pending correction too.  0 (nothing launched) when the group or the
  * configuration does not allow it: red-black with fused assembly, nswm
  * within one launch, nswp >= 1, and no outflow boundary (which rewrites
- * sc's ghost between the update and the sc solve). */
+ * sc's ghost between the update and the sc solve).  R (or null): the
+ * restriction that follows this down-leg relaxation; without periodic
+ * boundaries the update then does its restv too, and 2 is returned. */
 int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
-                                  SweepCfg cfg, int err, hipStream_t st);
+                                  SweepCfg cfg, int err, const DevPair *R, hipStream_t st);
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
                              hipStream_t st);
 /* the FP32 sweeps alone (they only touch the correction buffers); returns

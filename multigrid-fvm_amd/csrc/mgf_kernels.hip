@@ -344,12 +344,15 @@ __global__ void k_pstash(const DevLevel *Ls)
  * applied here (sc += d, as k_apply_mx would; the update does not read sc),
  * and the last p' launch left the updated p of every copy origin in tmp
  * (the value k_pstash would stash) */
+struct Cell4 {
+    real_t u, v, p, sc;
+};
+
+/* the update of cell (i, j), stored; returns the new u, v, p and sc */
 template <int PENDING>
-__global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
+__device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
 {
-    LEVEL(Ls);
     const DevPhys &P = L.P;
-    RANGE2D(1, L.npx - 2, 1, L.npy - 2);
     const int nx = L.npx;
     const int ij = j * nx + i;
     const real_t pp = L.pp[ij];
@@ -359,9 +362,11 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
         u = i <= nx - 3 ? u + (double)du : u;
         v = j <= L.npy - 3 ? v + (double)dv : v;
     }
-    L.u[ij] = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
-    L.v[ij] = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
-    if (PENDING == 2) L.sc[ij] = L.sc[ij] + (double)L.ds32[ij];
+    Cell4 c;
+    c.u = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
+    c.v = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
+    c.sc = L.sc[ij];
+    if (PENDING == 2) c.sc = c.sc + (double)L.ds32[ij];
     if (L.flags[ij] & MGF_F_PCOPY) {
         if (PENDING == 2) {
             /* the origin's updated p, stashed by the last p' launch; an
@@ -369,13 +374,60 @@ __global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
             const int o = L.porig[L.pslot[ij]];
             const int oi = o % nx, oj = o / nx;
             const bool in = (oi >= 1 && oi <= nx - 2 && oj >= 1 && oj <= L.npy - 2);
-            L.p[ij] = in ? L.tmp[o] : L.p[o];
+            c.p = in ? L.tmp[o] : L.p[o];
         } else {
-            L.p[ij] = L.pstash[L.pslot[ij]];
+            c.p = L.pstash[L.pslot[ij]];
         }
     } else {
-        L.p[ij] = L.p[ij] + P.relxp * (pp - 0.0);
+        c.p = L.p[ij] + P.relxp * (pp - 0.0);
     }
+    L.u[ij] = c.u;
+    L.v[ij] = c.v;
+    L.p[ij] = c.p;
+    if (PENDING == 2) L.sc[ij] = c.sc;
+    return c;
+}
+
+template <int PENDING>
+__global__ void __launch_bounds__(BX * BY) k_update(const DevLevel *Ls)
+{
+    LEVEL(Ls);
+    RANGE2D(1, L.npx - 2, 1, L.npy - 2);
+    (void)update_cell<PENDING>(L, i, j);
+}
+
+/* The update of a level followed at once by restv onto its next coarser
+ * level (nothing runs between them in a down-leg relaxation without
+ * periodic boundaries).  Threads take fine cells as k_update does; a 64 x 4
+ * workgroup holds whole 2 x 2 blocks (its rows start at an odd j, its
+ * columns at an odd i), so after an exchange through LDS the thread of each
+ * block's (2ic, 2jc) cell restricts the block with restv's expressions
+ * (V6:1370-1388).  The fine interior is exactly the blocks of the coarse
+ * interior. */
+template <int PENDING>
+__global__ void __launch_bounds__(BX * BY) k_update_restv(const DevLevel *Fs, const DevLevel *Cs)
+{
+    PAIR(Fs, Cs);
+    __shared__ real_t X[4][BY][BX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = 1 + (int)(T.x * BX + tx), j = 1 + (int)(T.y * BY + ty);
+    const bool in = i <= F.npx - 2 && j <= F.npy - 2;
+    if (T.y * BY + 1 > F.npy - 2) return;           /* padding rows: the whole workgroup */
+    Cell4 c{};
+    if (in) c = update_cell<PENDING>(F, i, j);
+    X[0][ty][tx] = c.u;
+    X[1][ty][tx] = c.v;
+    X[2][ty][tx] = c.p;
+    X[3][ty][tx] = c.sc;
+    __syncthreads();
+    if (!in || !(tx & 1) || !(ty & 1)) return;
+    /* this thread's cell is (2ic, 2jc): ij; its block's ij-1, ij-nxf, ij-nxf-1 */
+    const int ic = i / 2, jc = j / 2;
+    const int ijc = jc * C.npx + ic;
+    C.u[ijc] = (c.u * F.ax + X[0][ty - 1][tx] * F.ax) / (C.ax + 1.0e-30);
+    C.v[ijc] = (c.v * F.ay + X[1][ty][tx - 1] * F.ay) / (C.ay + 1.0e-30);
+    C.p[ijc] = 0.25 * (((c.p + X[2][ty][tx - 1]) + X[2][ty - 1][tx]) + X[2][ty - 1][tx - 1]);
+    C.sc[ijc] = 0.25 * (((c.sc + X[3][ty][tx - 1]) + X[3][ty - 1][tx]) + X[3][ty - 1][tx - 1]);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -839,11 +891,26 @@ void mgf_launch_outflow(const DevGroup &G, hipStream_t st)
                            dim3(256), 0, st, G.d);
 }
 
-void mgf_launch_restrict(const DevPair &R, hipStream_t st)
+void mgf_launch_restrict(const DevPair &R, int restv, hipStream_t st)
 {
     dim3 g = tile_grid(R.npxc - 2, R.npyc - 2, R.count);
-    hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, R.f, R.c);
+    if (restv) hipLaunchKernelGGL(k_restv, g, dim3(BX, BY), 0, st, R.f, R.c);
     hipLaunchKernelGGL(k_tau, g, dim3(BX, TAU_NT / BX), 0, st, R.f, R.c);
+}
+
+void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, hipStream_t st)
+{
+    if (G.max_npcopy > 0 && pending != 2)
+        hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
+                                          (unsigned)G.count),
+                           dim3(256), 0, st, G.d);
+    const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
+    if (pending == 2)
+        hipLaunchKernelGGL(k_update_restv<2>, g, dim3(BX, BY), 0, st, G.d, R.c);
+    else if (pending)
+        hipLaunchKernelGGL(k_update_restv<1>, g, dim3(BX, BY), 0, st, G.d, R.c);
+    else
+        hipLaunchKernelGGL(k_update_restv<0>, g, dim3(BX, BY), 0, st, G.d, R.c);
 }
 
 void mgf_launch_prolong(const DevPair &R, hipStream_t st)

@@ -647,7 +647,7 @@ static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
 }
 
 int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
-                                  SweepCfg cfg, int err, hipStream_t st)
+                                  SweepCfg cfg, int err, const DevPair *R, hipStream_t st)
 {
     const int lim = cfg.spl < MGF_MAX_FUSED_SWEEPS ? cfg.spl : MGF_MAX_FUSED_SWEEPS;
     if (isolve != 2 || !cfg.fuse || nswm < 1 || nswm > lim || nswp < 1 || G.any_outflow ||
@@ -660,6 +660,10 @@ int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int n
     }
     (void)launch_fused(G, 2, nswp, cfg, 1 | 2, st);
     if (err) mgf_launch_reduce_err(G, err == 2, st);
+    if (R && !G.any_pbound_x && !G.any_pbound_y) {
+        mgf_launch_update_restv(G, *R, 2, st);
+        return 2;
+    }
     mgf_launch_update(G, 2, st);
     return 1;
 }

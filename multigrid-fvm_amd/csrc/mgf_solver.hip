@@ -56,6 +56,7 @@ struct Program {
     DevLevel *d_desc = nullptr;           /* all group descriptor arrays */
     std::vector<DevGroup> groups;
     std::vector<DevPair> pairs;
+    std::vector<int> group_pair;          /* down-leg group -> its restriction (or -1) */
     std::vector<Op> ops;                  /* one V-cycle */
     hipGraphExec_t gexec = nullptr;
     int graph_ready = 0;
@@ -414,6 +415,12 @@ static int build_program(mgf_solver *s, Program &P, const std::vector<int> &memb
     HIPCHK(s, hipMemcpy(P.d_desc, host.data(), host.size() * sizeof(DevLevel),
                         hipMemcpyHostToDevice));
     P.ops = ops;
+    /* the restriction of each down-leg group: the pair with its key (same
+     * levels, same order) */
+    P.group_pair.assign(gm.size(), -1);
+    for (const auto &e : gkey)
+        if (e.first.kind == 0 && e.first.d >= 1)
+            P.group_pair[e.second] = find(pkey, Key{e.first.d, e.first.nx, e.first.ny, 2});
     if (&P == &s->full && s->nb == 1) {
         const Inst &I = s->inst[0];
         s->level_group.assign(I.ng, -1);
@@ -639,7 +646,8 @@ extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
  * returns it. */
 /* mixed precision: the same steps with FP64 assembly + residual, FP32
  * correction sweeps and FP64 apply (mgf_kernels_mixed.hip) */
-static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int err)
+static int enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int err,
+                               const DevPair *R)
 {
     hipStream_t st = s->st;
     const SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads,
@@ -649,9 +657,11 @@ static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int e
      * starts from), sc itself and its penalty / tau terms, none of which
      * the update or the periodic copies write; its correction is applied by
      * the update */
-    if (mgf_launch_relax_merged_mixed(G, s->isolve, s->nswpm, s->nswpp, cfg, err, st)) {
+    const int merged = mgf_launch_relax_merged_mixed(G, s->isolve, s->nswpm, s->nswpp, cfg, err,
+                                                     leg == 0 ? R : nullptr, st);
+    if (merged) {
         mgf_launch_pbound(G, st);
-        return;
+        return merged == 2;
     }
     mgf_launch_fluxes(G, st);
     mgf_launch_relax_field_mixed(G, 0, s->isolve, s->nswpm, cfg, 0, st);
@@ -666,14 +676,15 @@ static void enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int e
         mgf_launch_relax_field_mixed(G, 3, s->isolve, s->nswpm, cfg, 0, st);
         mgf_launch_outflow(G, st);
     }
+    return 0;
 }
 
-static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err)
+/* R: the restriction that follows a down-leg relaxation (or null); returns
+ * 1 when the relaxation did its restv as well */
+static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
+                         const DevPair *R = nullptr)
 {
-    if (s->opt.precision == MGF_PREC_MIXED) {
-        enqueue_relax_mixed(s, G, leg, err);
-        return;
-    }
+    if (s->opt.precision == MGF_PREC_MIXED) return enqueue_relax_mixed(s, G, leg, err, R);
     hipStream_t st = s->st;
     mgf_launch_fluxes(G, st);
     mgf_launch_asm_u(G, st);
@@ -693,17 +704,24 @@ static void enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err)
         mgf_launch_smooth(G, 3, s->isolve, s->nswpm, st);
         mgf_launch_outflow(G, st);
     }
+    return 0;
 }
 
 static void enqueue_vcycle(mgf_solver *s, const Program &P)
 {
+    /* pairs whose restv the preceding relaxation's update already did */
+    std::vector<char> restv_done(P.pairs.size(), 0);
     for (const Op &o : P.ops) {
         switch (o.kind) {
-        case OP_RELAX_DOWN:
-            enqueue_relax(s, P.groups[o.g], 0, P.groups[o.g].any_finest ? 2 : 0);
+        case OP_RELAX_DOWN: {
+            const DevGroup &G = P.groups[o.g];
+            const int pr = P.group_pair[o.g];
+            if (enqueue_relax(s, G, 0, G.any_finest ? 2 : 0, pr >= 0 ? &P.pairs[pr] : nullptr))
+                restv_done[pr] = 1;
             break;
+        }
         case OP_RELAX_UP: enqueue_relax(s, P.groups[o.g], 1, 0); break;
-        case OP_RESTRICT: mgf_launch_restrict(P.pairs[o.g], s->st); break;
+        case OP_RESTRICT: mgf_launch_restrict(P.pairs[o.g], !restv_done[o.g], s->st); break;
         case OP_PROLONG: mgf_launch_prolong(P.pairs[o.g], s->st); break;
         }
     }
@@ -798,7 +816,7 @@ extern "C" int mgf_restrict(mgf_solver *s, int ig)
     if (rc) return rc;
     if (ig < 0 || ig + 1 >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad fine level");
     if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
-    mgf_launch_restrict(s->full.pairs[s->level_pair[ig]], s->st);
+    mgf_launch_restrict(s->full.pairs[s->level_pair[ig]], 1, s->st);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;

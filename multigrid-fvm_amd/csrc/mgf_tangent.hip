@@ -111,7 +111,8 @@ void mgf_launch_relax_field_mixed(const DevGroup &, int, int, int, SweepCfg, int
     not_here("mgf_launch_relax_field_mixed");
 }
 
-int mgf_launch_relax_merged_mixed(const DevGroup &, int, int, int, SweepCfg, int, hipStream_t)
+int mgf_launch_relax_merged_mixed(const DevGroup &, int, int, int, SweepCfg, int, const DevPair *,
+                                  hipStream_t)
 {
     not_here("mgf_launch_relax_merged_mixed");
     return 0;
