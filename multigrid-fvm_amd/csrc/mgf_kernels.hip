@@ -496,16 +496,40 @@ __global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const Dev
     C.sc[ijc] = 0.25 * (((F.sc[ij] + F.sc[ijw]) + F.sc[ijs]) + F.sc[ijs - 1]);
 }
 
+/* where the residuals take u and v from, relative to their point (di, dj):
+ * the level's arrays, or a workgroup's LDS copy of them (row stride w) */
+struct StG {
+    const gdbl *u, *v;
+    int ij, nx;
+    __device__ real_t U(int di, int dj) const { return u[ij + dj * nx + di]; }
+    __device__ real_t V(int di, int dj) const { return v[ij + dj * nx + di]; }
+};
+struct StL {
+    const real_t *u, *v;
+    int o, w;
+    __device__ real_t U(int di, int dj) const { return u[o + dj * w + di]; }
+    __device__ real_t V(int di, int dj) const { return v[o + dj * w + di]; }
+};
+__device__ __forceinline__ StG st_g(const DevLevelG &L, int i, int j)
+{
+    return StG{L.u, L.v, j * L.npx + i, L.npx};
+}
+
 /* momentum residual at a u face with un-relaxed, un-penalised coefficients
- * (restru, V6:1496-1510 fine with resu, 1532-1545 coarse without) */
-__device__ __forceinline__ real_t rs_u(const DevLevelG &L, int i, int j, real_t body, bool fine,
-                                       bool mask = false)
+ * (restru, V6:1496-1510 fine with resu, 1532-1545 coarse without); the
+ * fluxes are fluxes' expression on the current u, v (coefu_at with
+ * FluxInl) */
+template <class S>
+__device__ __forceinline__ real_t rs_u(const DevLevelG &L, int i, int j, const S &s, real_t body,
+                                       bool fine, bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
-    Coef c = coefu_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
-    const gdbl *u = L.u;
-    real_t r = ((((c.aw * u[ij - 1] + c.ae * u[ij + 1]) + c.as * u[ij - nx]) + c.an * u[ij + nx]) -
-                c.ap * u[ij]) + L.ax * (L.p[ij] - L.p[ij + 1]);
+    const real_t ax = L.ax, ay = L.ay;
+    Coef c = coefu_vals(L, j, (0.5 * s.U(-1, 0)) * ax, (0.5 * s.U(0, 0)) * ax,
+                        (0.5 * s.U(1, 0)) * ax, (0.5 * s.V(0, -1)) * ay, (0.5 * s.V(1, -1)) * ay,
+                        (0.5 * s.V(0, 0)) * ay, (0.5 * s.V(1, 0)) * ay);
+    real_t r = ((((c.aw * s.U(-1, 0) + c.ae * s.U(1, 0)) + c.as * s.U(0, -1)) +
+                 c.an * s.U(0, 1)) - c.ap * s.U(0, 0)) + L.ax * (L.p[ij] - L.p[ij + 1]);
     if (fine) r = r + L.resu[ij];
     r = r + body;
     if (fine && L.ax == 0.0) r = 0.0;
@@ -513,14 +537,17 @@ __device__ __forceinline__ real_t rs_u(const DevLevelG &L, int i, int j, real_t 
     return r;
 }
 
-__device__ __forceinline__ real_t rs_v(const DevLevelG &L, int i, int j, real_t body, bool fine,
-                                       bool mask = false)
+template <class S>
+__device__ __forceinline__ real_t rs_v(const DevLevelG &L, int i, int j, const S &s, real_t body,
+                                       bool fine, bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
-    Coef c = coefv_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j);
-    const gdbl *v = L.v;
-    real_t r = ((((c.aw * v[ij - 1] + c.ae * v[ij + 1]) + c.as * v[ij - nx]) + c.an * v[ij + nx]) -
-                c.ap * v[ij]) + L.ay * (L.p[ij] - L.p[ij + nx]);
+    const real_t ax = L.ax, ay = L.ay;
+    Coef c = coefv_vals(L, i, (0.5 * s.U(-1, 0)) * ax, (0.5 * s.U(-1, 1)) * ax,
+                        (0.5 * s.U(0, 0)) * ax, (0.5 * s.U(0, 1)) * ax, (0.5 * s.V(0, -1)) * ay,
+                        (0.5 * s.V(0, 0)) * ay, (0.5 * s.V(0, 1)) * ay);
+    real_t r = ((((c.aw * s.V(-1, 0) + c.ae * s.V(1, 0)) + c.as * s.V(0, -1)) +
+                 c.an * s.V(0, 1)) - c.ap * s.V(0, 0)) + L.ay * (L.p[ij] - L.p[ij + nx]);
     if (fine) r = r + L.resv[ij];
     r = r + body;
     if (fine && L.ay == 0.0) r = 0.0;
@@ -528,16 +555,19 @@ __device__ __forceinline__ real_t rs_v(const DevLevelG &L, int i, int j, real_t 
     return r;
 }
 
-__device__ __forceinline__ real_t rs_sc(const DevLevelG &L, int i, int j, bool fine,
+template <class S>
+__device__ __forceinline__ real_t rs_sc(const DevLevelG &L, int i, int j, const S &st, bool fine,
                                         bool mask = false)
 {
     const int nx = L.npx, ij = j * nx + i;
-    Coef c = coefsc_at(L, FluxInl{L.u, L.v, L.ax, L.ay}, i, j, L.flags[ij]);
+    const uint8_t fl = L.flags[ij];
+    Coef c = coefsc_vals(L, ij, fl, (0.5 * st.U(-1, 0)) * L.ax, (0.5 * st.U(0, 0)) * L.ax,
+                         (0.5 * st.V(0, -1)) * L.ay, (0.5 * st.V(0, 0)) * L.ay);
     const gdbl *s = L.sc;
     real_t r = (((c.aw * s[ij - 1] + c.ae * s[ij + 1]) + c.as * s[ij - nx]) + c.an * s[ij + nx]) -
                c.ap * s[ij];
     if (fine) r = r + L.ressc[ij];
-    if (mask && (L.flags[ij] & MGF_F_SPSC)) r = 0.0;
+    if (mask && (fl & MGF_F_SPSC)) r = 0.0;
     return r;
 }
 
@@ -546,24 +576,51 @@ __device__ __forceinline__ real_t rs_sc(const DevLevelG &L, int i, int j, bool f
  * evaluates every fine residual they need once, at unit stride, into LDS
  * (the reference's rs scratch array, V6:1496-1530, restricted to the
  * workgroup's footprint), then each coarse point combines its fine
- * residuals in the reference's order and subtracts its coarse residual. */
+ * residuals in the reference's order and subtracts its coarse residual.
+ * The fine u and v the residual stencils reach (fine columns fi0-1 ..
+ * fi0+2BX+1, rows fj0-1 .. fj0+2BY+1) are first copied to LDS at unit
+ * stride, so each is loaded once per workgroup instead of by every stencil
+ * that reaches it (FP64 only: dual numbers would not fit). */
 #define TAU_UW (2 * BX + 1)
 #define TAU_W (2 * BX)
 #define TAU_H (2 * BY)
+#define TAU_SW (2 * BX + 3)
+#define TAU_SH (2 * BY + 3)
 /* threads of a k_tau workgroup: more than the 64 x 4 coarse points, so each
  * thread evaluates fewer fine residuals one after another */
 #define TAU_NT (2 * BX * BY)
 __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
+    constexpr bool STAGE = sizeof(real_t) == sizeof(double);
+    constexpr int NS = STAGE ? TAU_SH * TAU_SW : 1;
     __shared__ real_t U[TAU_H][TAU_UW];
     __shared__ real_t V[TAU_H + 1][TAU_W];
     __shared__ real_t Q[TAU_H][TAU_W];
+    __shared__ real_t Su[NS], Sv[NS];
     const int nxc = C.npx, nyc = C.npy;
     const int ic0 = 1 + T.x * BX, jc0 = 1 + T.y * BY;
     if (ic0 > nxc - 2 || jc0 > nyc - 2) return;          /* padding tiles */
     const int fi0 = 2 * ic0 - 1, fj0 = 2 * jc0 - 1;
     const int tid = threadIdx.y * BX + threadIdx.x;
+    if (STAGE) {
+        const int nxf = F.npx, nyf = F.npy;
+        for (int e = tid; e < TAU_SH * TAU_SW; e += TAU_NT) {
+            const int r = e / TAU_SW, c = e - r * TAU_SW;
+            const int gi = fi0 - 1 + c, gj = fj0 - 1 + r;
+            const bool ok = gi <= nxf - 1 && gj <= nyf - 1;
+            Su[e] = ok ? F.u[gj * nxf + gi] : 0.0;
+            Sv[e] = ok ? F.v[gj * nxf + gi] : 0.0;
+        }
+        __syncthreads();
+    }
+    /* the fine point (fi0 + c, fj0 + r)'s access to u and v */
+    auto fst = [&](int c, int r) {
+        if constexpr (STAGE)
+            return StL{Su, Sv, (r + 1) * TAU_SW + (c + 1), TAU_SW};
+        else
+            return st_g(F, fi0 + c, fj0 + r);
+    };
     /* onto a level below the obstacle level the penalised fine residuals
      * are dropped (their nonzero value is balanced by the 1e30 penalty,
      * which the residual leaves out); never the case when ig_obs == ngrid */
@@ -577,15 +634,17 @@ __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLev
     const int cq = 2 * (icc - ic0) + 1, rq = 2 * (jcc - jc0) + 1;
     for (int e = tid; e < TAU_H * TAU_UW; e += TAU_NT) {
         const int r = e / TAU_UW, c = e - r * TAU_UW;
-        U[r][c] = (c <= cu && r <= ru) ? rs_u(F, fi0 + c, fj0 + r, F.bodyx, true, m) : 0.0;
+        U[r][c] = (c <= cu && r <= ru) ? rs_u(F, fi0 + c, fj0 + r, fst(c, r), F.bodyx, true, m)
+                                       : 0.0;
     }
     for (int e = tid; e < (TAU_H + 1) * TAU_W; e += TAU_NT) {
         const int r = e / TAU_W, c = e - r * TAU_W;
-        V[r][c] = (c <= cv && r <= rv) ? rs_v(F, fi0 + c, fj0 + r, F.bodyy, true, m) : 0.0;
+        V[r][c] = (c <= cv && r <= rv) ? rs_v(F, fi0 + c, fj0 + r, fst(c, r), F.bodyy, true, m)
+                                       : 0.0;
     }
     for (int e = tid; e < TAU_H * TAU_W; e += TAU_NT) {
         const int r = e / TAU_W, c = e - r * TAU_W;
-        Q[r][c] = (c <= cq && r <= rq) ? rs_sc(F, fi0 + c, fj0 + r, true, m) : 0.0;
+        Q[r][c] = (c <= cq && r <= rq) ? rs_sc(F, fi0 + c, fj0 + r, fst(c, r), true, m) : 0.0;
     }
     __syncthreads();
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -597,20 +656,20 @@ __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLev
         real_t a = U[r1][c1], b = U[r1 - 1][c1];
         real_t w0 = U[r1][c1 - 1], w1 = U[r1 - 1][c1 - 1];
         real_t e0 = U[r1][c1 + 1], e1 = U[r1 - 1][c1 + 1];
-        real_t rc = rs_u(C, ic, jc, C.bodyx, false);
+        real_t rc = rs_u(C, ic, jc, st_g(C, ic, jc), C.bodyx, false);
         C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rc;
     }
     if (jc <= nyc - 3) {
         real_t a = V[r1][c1], b = V[r1][c1 - 1];
         real_t s0 = V[r1 - 1][c1], s1 = V[r1 - 1][c1 - 1];
         real_t n0 = V[r1 + 1][c1], n1 = V[r1 + 1][c1 - 1];
-        real_t rc = rs_v(C, ic, jc, C.bodyy, false);
+        real_t rc = rs_v(C, ic, jc, st_g(C, ic, jc), C.bodyy, false);
         C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rc;
     }
     {
         real_t a = Q[r1][c1], b = Q[r1 - 1][c1];
         real_t c2 = Q[r1][c1 - 1], d = Q[r1 - 1][c1 - 1];
-        real_t rc = rs_sc(C, ic, jc, false);
+        real_t rc = rs_sc(C, ic, jc, st_g(C, ic, jc), false);
         C.ressc[ijc] = (((a + b) + c2) + d) - rc;
     }
 }
