@@ -48,11 +48,11 @@ sys.path.insert(0, os.path.join(ROOT, "multigrid-fvm_amd"))
 METRIC = ("converged solves/s at FP64 residual 1e-6, batched-mixed vs plain GPU; "
           "% HBM peak")
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the roofline kernel (k_rb32x<2,256,1,0> over the 84
+# HBM bytes per launch of the roofline kernel (k_rb32x_uvs<2,256> over the 84
 # 512^2 members): rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # `bench.py --roofline-only --reps 10`, each scaled by what the counter reads
-# for a 1 GiB device copy (profiles/traffic_relax_u_512x84.md)
-TRAFFIC = 1425.6e6
+# for a 1 GiB device copy (profiles/traffic_relax_uvs_512x84.md)
+TRAFFIC = 2084.2e6
 WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
             "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
             "each obstacle moved by a seeded quarter/half/full cell), Re in "
@@ -101,14 +101,15 @@ def time_reference_cycle(prob, workdir, target_s=4.0):
 
 
 def kernel_roofline(probs, pairs, reps, cycles=None):
-    """The dominant kernels of a batched mixed V-cycle are the fused
-    relaxation launches of u / v / sc (k_rb32x<2,256,1,field>: FP64 assembly
-    of the equation, two FP32 red-black sweeps, FP32 correction out).  Its mean duration is
-    timed with HIP events on the batch's own stream for the u launch over
-    every member of the largest grid size at level 0, after that batch has
-    run (graph-replayed V-cycles cannot be bracketed per kernel, so the
-    launch is repeated on its own).  Also reported: the FP32 sweep kernel
-    (k_rb32x<3,512,0,-1>, three p' sweeps on the stored equation)."""
+    """The dominant kernel of a batched mixed V-cycle is the launch that
+    relaxes u, v and sc together (k_rb32x_uvs<2,256>: for each field, FP64
+    assembly of its equation on the tile and halo, two FP32 red-black sweeps,
+    FP32 correction out).  Its mean duration is timed with HIP events on the
+    batch's own stream over every member of the largest grid size at level 0,
+    after that batch has run (graph-replayed V-cycles cannot be bracketed per
+    kernel, so the launch is repeated on its own).  Also reported: the FP32
+    sweep kernel (k_rb32x<3,512,0,-1>, three p' sweeps on the stored
+    equation)."""
     import mgfvm
     nmax = max(p.nfx for p in probs)
     big = [k for k in range(len(probs)) if probs[k].nfx == nmax]
@@ -120,17 +121,18 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
     bb = mgfvm.Batch(big_pairs, precision=mgfvm.PREC_MIXED, device=0)
     bb.reset()
     bb.solve()          # leaves an assembled state behind
-    us, nbytes = bb.time_kernel(mgfvm.KERNEL_RELAX_U, 0, reps)
+    us, nbytes = bb.time_kernel(mgfvm.KERNEL_RELAX_UVS, 0, reps)
     us_s, nbytes_s = bb.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, reps)
     bb.close()
     achieved = nbytes / (us * 1e-6) / 1e9
-    nunk = (nmax - 1) * nmax * len(big_pairs)
+    ncell = nmax * nmax * len(big_pairs)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
-            "kernel": f"k_rb32x<2,256,1,0>: fused u relaxation launch (FP64 assembly + 2 FP32 "
-                      f"red-black sweeps) over the {len(big_pairs)} {nmax}^2 members, level 0",
+            "kernel": f"k_rb32x_uvs<2,256>: u, v and sc relaxed in one launch (FP64 assembly + "
+                      f"2 FP32 red-black sweeps each) over the {len(big_pairs)} {nmax}^2 "
+                      f"members, level 0",
             "mean_us": round(us, 2), "bytes_per_launch": nbytes,
-            "bytes_per_unknown": round(nbytes / nunk, 2),
+            "bytes_per_cell": round(nbytes / ncell, 2),
             "sweep_kernel": {"kernel": "k_rb32x<3,512,0,-1>: 3 FP32 red-black p' sweeps on "
                                        "the stored equation, same members",
                              "mean_us": round(us_s, 2), "bytes_per_launch": nbytes_s,

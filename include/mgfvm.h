@@ -238,6 +238,9 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
 #define MGF_KERNEL_RELAX_U   2   /* mixed red-black: the first launch of the u
                                     relaxation (assembly + sweeps in one kernel) */
 #define MGF_KERNEL_RELAX_PP  3   /* the same for p' */
+#define MGF_KERNEL_RELAX_UVS 4   /* mixed red-black: the launch that relaxes u, v and
+                                    sc together (each assembled, nswpm sweeps), as
+                                    a V-cycle issues it */
 /* Time `reps` smoother launches of level ig (for a batch: the launch
  * covering every problem's level at the depth of problem 0's level ig) with
  * HIP events on the solver's stream, on the current coefficients.  A launch
@@ -252,7 +255,10 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
  * equation and runs the first min(sweeps_per_launch, nswpm or nswpp)
  * sweeps; its bytes are the compulsory ones per unknown (the FP64 state the
  * assembly reads, the flag byte, the FP32 correction or pp written, apu /
- * apv, and the FP32 equation when later launches reuse it). */
+ * apv, and the FP32 equation when later launches reuse it).
+ * MGF_KERNEL_RELAX_UVS: bytes per interior cell are the union of what its
+ * three equations must move: u, v, p, sc, resu, resv, ressc read (FP64),
+ * the flag byte, the three FP32 corrections and apu, apv written. */
 int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps,
                     double *mean_us, double *bytes_per_launch);
 

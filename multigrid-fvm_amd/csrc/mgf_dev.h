@@ -202,6 +202,9 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
  * boundaries the update then does its restv too, and 2 is returned. */
 int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
                                   SweepCfg cfg, int err, const DevPair *R, hipStream_t st);
+/* its first launch alone (u, v and sc; for timing: it rewrites what it
+ * wrote before, given unchanged inputs); 0 when not applicable */
+int mgf_launch_uvs_mixed(const DevGroup &G, int isolve, int nswm, SweepCfg cfg, hipStream_t st);
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
                              hipStream_t st);
 /* the FP32 sweeps alone (they only touch the correction buffers); returns

@@ -646,18 +646,24 @@ static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
         hipLaunchKernelGGL((k_rb32x_uvs<S, 256>), g, dim3(BX, 4), 0, st, G.d);
 }
 
-int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
-                                  SweepCfg cfg, int err, const DevPair *R, hipStream_t st)
+int mgf_launch_uvs_mixed(const DevGroup &G, int isolve, int nswm, SweepCfg cfg, hipStream_t st)
 {
     const int lim = cfg.spl < MGF_MAX_FUSED_SWEEPS ? cfg.spl : MGF_MAX_FUSED_SWEEPS;
-    if (isolve != 2 || !cfg.fuse || nswm < 1 || nswm > lim || nswp < 1 || G.any_outflow ||
-        G.npx - 3 < 1 || G.npy - 3 < 1)
+    if (isolve != 2 || !cfg.fuse || nswm < 1 || nswm > lim || G.any_outflow || G.npx - 3 < 1 ||
+        G.npy - 3 < 1)
         return 0;
     switch (nswm) {
     case 1: launch_uvs_s<1>(G, cfg.nt_asm, st); break;
     case 2: launch_uvs_s<2>(G, cfg.nt_asm, st); break;
     default: launch_uvs_s<3>(G, cfg.nt_asm, st); break;
     }
+    return 1;
+}
+
+int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
+                                  SweepCfg cfg, int err, const DevPair *R, hipStream_t st)
+{
+    if (nswp < 1 || !mgf_launch_uvs_mixed(G, isolve, nswm, cfg, st)) return 0;
     (void)launch_fused(G, 2, nswp, cfg, 1 | 2, st);
     if (err) mgf_launch_reduce_err(G, err == 2, st);
     if (R && !G.any_pbound_x && !G.any_pbound_y) {

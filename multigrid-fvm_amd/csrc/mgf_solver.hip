@@ -1090,10 +1090,11 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
 {
     if (!s || reps < 1) return set_err(s, MGF_ERR_ARG, "bad arguments");
     if (ig < 0 || ig >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad level");
-    if (kernel < MGF_KERNEL_SWEEP_PP || kernel > MGF_KERNEL_RELAX_PP)
+    if (kernel < MGF_KERNEL_SWEEP_PP || kernel > MGF_KERNEL_RELAX_UVS)
         return set_err(s, MGF_ERR_ARG, "unknown kernel id");
     const int mixed = s->opt.precision == MGF_PREC_MIXED;
-    const int fused = kernel == MGF_KERNEL_RELAX_U || kernel == MGF_KERNEL_RELAX_PP;
+    const int uvs = kernel == MGF_KERNEL_RELAX_UVS;
+    const int fused = kernel == MGF_KERNEL_RELAX_U || kernel == MGF_KERNEL_RELAX_PP || uvs;
     if (fused && (!mixed || s->isolve != 2 || s->opt.split_assembly))
         return set_err(s, MGF_ERR_UNSUPPORTED,
                        "fused relaxation launches exist for mixed-precision red-black only");
@@ -1132,8 +1133,11 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     const int spl = (mixed && isolve == 2) ? cfg.spl : 1;
     const SweepCfg fcfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads, 0);
     double fused_bpu = 0.0;
+    int uvs_ok = 1;
     auto launch = [&](int nl) {
-        if (fused) {
+        if (uvs) {
+            for (int k = 0; k < nl; ++k) uvs_ok &= mgf_launch_uvs_mixed(G, isolve, s->nswpm, fcfg, s->st);
+        } else if (fused) {
             int nf = 0;
             for (int k = 0; k < nl; ++k)
                 fused_bpu = mgf_launch_relax_first_mixed(G, field, nsw_field, fcfg, &nf, s->st);
@@ -1161,6 +1165,15 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
         (void)hipFree(save);
     }
     if ((rc = check_launch(s))) return rc;
+    if (!uvs_ok)
+        return set_err(s, MGF_ERR_UNSUPPORTED,
+                       "no merged u, v, sc launch for this level or configuration");
+    if (uvs) {
+        const double ncell = (double)(G.npx - 2) * (double)(G.npy - 2) * (double)G.count;
+        if (mean_us) *mean_us = 1000.0 * (double)ms / (double)even;
+        if (bytes_per_launch) *bytes_per_launch = (7 * 8.0 + 1.0 + 3 * 4.0 + 2 * 8.0) * ncell;
+        return MGF_OK;
+    }
     int nx = field == 2 ? G.npx - 1 : G.npx - 2, ny = G.npy - 1;
     const double nunk = (double)(nx - 1) * (double)(ny - 1) * (double)G.count;
     /* algorithmic words per unknown and sweep: aw ae as an ap su and phi read

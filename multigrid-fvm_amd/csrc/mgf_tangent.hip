@@ -118,6 +118,12 @@ int mgf_launch_relax_merged_mixed(const DevGroup &, int, int, int, SweepCfg, int
     return 0;
 }
 
+int mgf_launch_uvs_mixed(const DevGroup &, int, int, SweepCfg, hipStream_t)
+{
+    not_here("mgf_launch_uvs_mixed");
+    return 0;
+}
+
 int mgf_launch_sweeps_mixed(const DevGroup &, int, int, int, SweepCfg, hipStream_t)
 {
     not_here("mgf_launch_sweeps_mixed");

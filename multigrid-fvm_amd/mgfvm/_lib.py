@@ -33,6 +33,7 @@ KERNEL_SWEEP_PP = 0
 KERNEL_SWEEP_U = 1
 KERNEL_RELAX_U = 2
 KERNEL_RELAX_PP = 3
+KERNEL_RELAX_UVS = 4
 
 
 class MgfError(RuntimeError):
