@@ -54,6 +54,21 @@ int mgf_tangent_get_field(mgf_solver *s, int b, int ig, const char *name, double
  *          sc of the last interior cell (outlet scalar flux, upwind). */
 int mgf_tangent_objectives(mgf_solver *s, int b, double J[2], double dJ[2]);
 
+/* The whole gradient of J with respect to the inlet profile (rows 1..nfy),
+ * amuf and prsc: the nfy + 2 directions of problem p (its inlet profile as
+ * given in p->inlet_u), solved as batches of at most max_batch problems
+ * (0: all in one batch).  An inlet-row direction also raises the east
+ * boundary u by 1/nfy (see mgf_seed), so it is the derivative along a
+ * mass-consistent change.  Outputs: J[2]; g_inlet[2 * nfy] with
+ * dJ[k]/d(inlet row r) at k * nfy + (r - 1); g_amuf[2]; g_prsc[2];
+ * *converged = 1 when every direction's solve met its tolerance (any output
+ * may be NULL).  This is the gradient entry the reference's design lists as
+ * an adjoint call (the reference has no derivative capability); here it is
+ * computed in forward mode, nfy + 2 solves batched on the device. */
+int mgf_tangent_gradient(const mgf_problem *p, const mgf_options *opt, int max_batch,
+                         double J[2], double *g_inlet, double g_amuf[2], double g_prsc[2],
+                         int *converged);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+
+#include <algorithm>
 #include <vector>
 
 #include "mgf_dev.h"
@@ -83,6 +85,62 @@ extern "C" int mgf_tangent_objectives(mgf_solver *s, int b, double J[2], double 
     dJ[0] = j1.d;
     dJ[1] = j2.d;
     return MGF_OK;
+}
+
+extern "C" int mgf_tangent_gradient(const mgf_problem *p, const mgf_options *opt, int max_batch,
+                                    double J[2], double *g_inlet, double g_amuf[2],
+                                    double g_prsc[2], int *converged)
+{
+    if (!p || max_batch < 0 || p->nfy < 1) {
+        mgf_set_error(NULL, MGF_ERR_ARG, "bad arguments");
+        return MGF_ERR_ARG;
+    }
+    const int nfy = p->nfy, nd = nfy + 2;
+    const int chunk = max_batch > 0 && max_batch < nd ? max_batch : nd;
+    std::vector<double> unit((size_t)chunk * nfy);
+    std::vector<mgf_seed> seeds(chunk);
+    std::vector<const mgf_problem *> probs(chunk, p);
+    std::vector<int> ncyc(chunk), conv(chunk);
+    int all_conv = 1, rc = MGF_OK;
+    /* directions 0 .. nfy-1: inlet rows 1 .. nfy; nfy: amuf; nfy + 1: prsc */
+    for (int d0 = 0; d0 < nd && rc == MGF_OK; d0 += chunk) {
+        const int n = nd - d0 < chunk ? nd - d0 : chunk;
+        std::fill(unit.begin(), unit.end(), 0.0);
+        for (int b = 0; b < n; ++b) {
+            const int d = d0 + b;
+            mgf_seed &sd = seeds[b];
+            sd = mgf_seed{0.0, 0.0, NULL, 0.0};
+            if (d < nfy) {
+                unit[(size_t)b * nfy + d] = 1.0;
+                sd.d_inlet_u = &unit[(size_t)b * nfy];
+                sd.d_u_xp = 1.0 / (double)nfy;
+            } else if (d == nfy) {
+                sd.d_amuf = 1.0;
+            } else {
+                sd.d_prsc = 1.0;
+            }
+        }
+        mgf_solver *s = NULL;
+        if ((rc = mgf_tangent_batch_create(probs.data(), seeds.data(), n, opt, &s))) break;
+        rc = mgf_batch_solve(s, ncyc.data(), conv.data());
+        for (int b = 0; b < n && rc == MGF_OK; ++b) {
+            double Jb[2], dJ[2];
+            if ((rc = mgf_tangent_objectives(s, b, Jb, dJ))) break;
+            const int d = d0 + b;
+            if (d == 0 && J) { J[0] = Jb[0]; J[1] = Jb[1]; }
+            if (d < nfy && g_inlet) { g_inlet[d] = dJ[0]; g_inlet[nfy + d] = dJ[1]; }
+            if (d == nfy && g_amuf) { g_amuf[0] = dJ[0]; g_amuf[1] = dJ[1]; }
+            if (d == nfy + 1 && g_prsc) { g_prsc[0] = dJ[0]; g_prsc[1] = dJ[1]; }
+            all_conv &= conv[b];
+        }
+        if (rc) {
+            /* keep the solver's message for mgf_error_string */
+            mgf_set_error(NULL, rc, mgf_last_error(s));
+        }
+        mgf_destroy(s);
+    }
+    if (converged) *converged = rc == MGF_OK && all_conv;
+    return rc;
 }
 
 /* ---- mixed-precision launchers: not part of this library --------------- */

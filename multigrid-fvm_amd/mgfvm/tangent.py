@@ -20,7 +20,8 @@ from ._lib import PKG_DIR, InletProfile, MgfError, Options, ProblemStruct
 
 TAN_PATH = os.path.join(PKG_DIR, "libmgfvm_tangent.so")
 
-EXPORTS = ["mgf_tangent_batch_create", "mgf_tangent_get_field", "mgf_tangent_objectives"]
+EXPORTS = ["mgf_tangent_batch_create", "mgf_tangent_get_field", "mgf_tangent_objectives",
+           "mgf_tangent_gradient"]
 
 
 class Seed(ctypes.Structure):
@@ -54,6 +55,8 @@ def tlib():
                                                ctypes.POINTER(Options), ctypes.POINTER(vp)]
         L.mgf_tangent_get_field.argtypes = [vp, ip, ip, cp, P_d, P_d]
         L.mgf_tangent_objectives.argtypes = [vp, ip, P_d, P_d]
+        L.mgf_tangent_gradient.argtypes = [PP, ctypes.POINTER(Options), ip, P_d, P_d, P_d, P_d,
+                                           P_i]
         _tlib = L
     return _tlib
 
@@ -174,6 +177,38 @@ def gradients(inputdata: str, pattern: str, inlet_u=None, rows: Optional[Sequenc
     J = outs[0][0]
     dJ = np.array([o[1] for o in outs])            # directions x 2
     return J, dJ[:len(rows)].T, dJ[len(rows)], dJ[len(rows) + 1], all(conv)
+
+
+def gradient_c(inputdata: str, pattern: str, inlet_u=None, max_batch: int = 0,
+               device: int = -1):
+    """The same gradient through the library's own entry point,
+    mgf_tangent_gradient (every inlet row, amuf, prsc; batches of at most
+    max_batch directions, 0 = one batch).  Returns (J, dJ_dinlet [2 x nfy],
+    dJ_damuf [2], dJ_dprsc [2], converged)."""
+    L = tlib()
+    p = ctypes.POINTER(ProblemStruct)()
+    rc = L.mgf_problem_read(inputdata.encode(), pattern.encode(), ctypes.byref(p))
+    if rc != 0:
+        raise MgfError(rc, L.mgf_error_string().decode())
+    try:
+        nfy = p.contents.nfy
+        opt = Options()
+        L.mgf_options_default(ctypes.byref(opt))
+        opt.device = device
+        J = (ctypes.c_double * 2)()
+        gi = (ctypes.c_double * (2 * nfy))()
+        ga = (ctypes.c_double * 2)()
+        gp = (ctypes.c_double * 2)()
+        conv = ctypes.c_int()
+        with InletProfile(p, inlet_u):
+            rc = L.mgf_tangent_gradient(p, ctypes.byref(opt), max_batch, J, gi, ga, gp,
+                                        ctypes.byref(conv))
+        if rc != 0:
+            raise MgfError(rc, L.mgf_error_string().decode())
+        return (np.array(J[:]), np.array(gi[:]).reshape(2, nfy), np.array(ga[:]),
+                np.array(gp[:]), bool(conv.value))
+    finally:
+        L.mgf_problem_free(p)
 
 
 def _problem_files(problem, inlet_u, amuf, prsc, directory):

@@ -177,3 +177,26 @@ def test_torch_function_backward(tmp_path):
     np.testing.assert_allclose(x.grad.numpy(), w @ din, rtol=1e-14)
     np.testing.assert_allclose(mu.grad.item(), w @ dmu, rtol=1e-14)
     np.testing.assert_allclose(pr.grad.item(), w @ dpr, rtol=1e-14)
+
+
+def test_c_gradient_entry_matches_batched_directions(tmp_path):
+    """mgf_tangent_gradient (the library's own gradient call: every inlet
+    row, amuf and prsc, in batches of max_batch directions) gives the bits
+    of the same directions solved as one Python-built batch, whatever the
+    batch size; rows are mass-consistent directions (east u += 1/nfy)."""
+    from mgfvm.tangent import gradient_c, gradients
+    nf = 32
+    inlet = _inlet(nf, seed=5)
+    prob = Problem(nfx=nf, nfy=nf, ngrid=3, ig_obs=3, nvcycle=3000, convergence=1e-12,
+                   isolve=2, nswpm=2, nswpp=6, amuf=0.01,
+                   bc_xp=(float(inlet.mean()), 0.0, 0.0, 0.0),
+                   obstacles=random_mask(8, 0.85, 4))
+    pin, ppat = prob.write(str(tmp_path / "g"))
+    _say("[gpu] mgf_tangent_gradient vs batched directions")
+    ref = gradients(pin, ppat, inlet_u=inlet)
+    assert ref[4]
+    for mb in (0, 7):
+        got = gradient_c(pin, ppat, inlet_u=inlet, max_batch=mb)
+        assert got[4], mb
+        for a, b in zip(got[:4], ref[:4]):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), mb
