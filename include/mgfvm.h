@@ -98,6 +98,11 @@ typedef struct mgf_options {
                                   a kernel of its own that stores it in FP32; 0 (default)
                                   = the first sweep launch assembles it on the fly.
                                   Result-neutral. */
+    int gs_order;              /* isolve 2 sweeps: 0 (default) red-black, the reference's
+                                  solvergs (V6:1297-1351); 1 lexicographic Gauss-Seidel,
+                                  a labelled variant the reference does not have: its
+                                  loops without the colour tests (j outer, i inner), run
+                                  as anti-diagonal wavefronts.  FP64 only, nfy <= 1024. */
     int compact;               /* batched solves with use_graph: once enough instances
                                   have stopped, capture the V-cycle again over the live
                                   ones only (1, default), instead of launching over all

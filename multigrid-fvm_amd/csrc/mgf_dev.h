@@ -149,7 +149,9 @@ void mgf_launch_asm_v(const DevGroup &G, hipStream_t st);
  * level-0 members (2), or neither (0) */
 void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st);
 void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
-/* field: 0 u, 1 v, 2 pp, 3 sc */
+/* field: 0 u, 1 v, 2 pp, 3 sc; isolve 1 Jacobi, 2 red-black, 3 lexicographic
+ * Gauss-Seidel (the labelled variant; rows <= LEX_MAX_ROWS) */
+#define LEX_MAX_ROWS 1024
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1),
  * and the sc correction in ds32 as well, with the updated p of the copy

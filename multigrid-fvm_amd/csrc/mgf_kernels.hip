@@ -897,6 +897,63 @@ void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st)
     hipLaunchKernelGGL(k_asm_sc, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
 }
 
+/* Lexicographic Gauss-Seidel: the labelled variant of isolve 2 that this
+ * library adds (the reference has only red-black, V6:1297-1351): nsw
+ * in-place passes with j outer and i inner, each point updated with
+ * solvergs's expression.  Run as anti-diagonal wavefronts, one workgroup per
+ * problem: thread k owns row r = k + 1 and updates column c = t - k + 1 at
+ * step t, so each point sees exactly the values the sequential loop gives
+ * it -- west and south already updated this sweep (its own previous result,
+ * and the row below's result of the previous step, passed through LDS), east
+ * and north not yet (read from the field before their owners reach them:
+ * the next column's operands are loaded one step ahead, and a workgroup's
+ * accesses are ordered across its barriers).  Rows <= LEX_MAX_ROWS. */
+struct LexOp {
+    real_t aw, ae, as, an, ap, su, e, n;
+};
+
+__device__ __forceinline__ LexOp lex_op(const DevLevelG &L, const gdbl *phi, int ij, int npx)
+{
+    return LexOp{L.aw[ij], L.ae[ij], L.as[ij], L.an[ij], L.ap[ij], L.su[ij], phi[ij + 1],
+                 phi[ij + npx]};
+}
+
+__global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int field, int nsw)
+{
+    const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[blockIdx.z];
+    if (L.ctl->done) return;
+    int nx, ny;
+    gdbl *phi = field_of(L, field, nx, ny);
+    const int NC = nx - 1, NR = ny - 1, npx = L.npx;
+    __shared__ real_t sbuf[2][LEX_MAX_ROWS];
+    const int k = threadIdx.x, r = k + 1;
+    const bool row = r <= NR;
+    const int base = r * npx;
+    for (int it = 0; it < nsw; ++it) {
+        real_t west = 0.0;
+        LexOp op{};
+        if (row) {
+            west = phi[base];
+            op = lex_op(L, phi, base + 1, npx);
+        }
+        for (int t = 0; t < NC + NR - 1; ++t) {
+            const int c = t - k + 1;
+            if (row && c >= 1 && c <= NC) {
+                const int ij = base + c;
+                const real_t south = k == 0 ? (real_t)phi[ij - npx] : sbuf[(t - 1) & 1][k - 1];
+                const real_t s =
+                    ((op.aw * west + op.ae * op.e) + op.as * south) + op.an * op.n;
+                const real_t x = (s + op.su) / op.ap;
+                phi[ij] = x;
+                west = x;
+                sbuf[t & 1][k] = x;
+                if (c < NC) op = lex_op(L, phi, ij + 1, npx);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
 {
     int nx, ny;
@@ -905,7 +962,13 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
     case 1: nx = G.npx - 1; ny = G.npy - 2; break;
     default: nx = G.npx - 1; ny = G.npy - 1; break;
     }
-    if (nx - 1 < 1 || ny - 1 < 1 || nsw <= 0 || (isolve != 1 && isolve != 2)) return;
+    if (nx - 1 < 1 || ny - 1 < 1 || nsw <= 0 || isolve < 1 || isolve > 3) return;
+    if (isolve == 3) {
+        const unsigned nt = (unsigned)((G.npy - 2 + 63) / 64 * 64);
+        hipLaunchKernelGGL(k_lexgs, dim3(1, 1, (unsigned)G.count), dim3(nt), 0, st, G.d, field,
+                           nsw);
+        return;
+    }
     for (int it = 0; it < nsw; ++it) {
         if (isolve == 1)
             hipLaunchKernelGGL(k_jacobi, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,

@@ -35,6 +35,7 @@ void mgf_options_default(mgf_options *o)
     o->sweeps_per_launch = 0;
     o->sweep_threads = 0;
     o->split_assembly = 0;
+    o->gs_order = 0;
     o->compact = 1;
 }
 

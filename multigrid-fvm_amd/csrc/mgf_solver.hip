@@ -69,6 +69,7 @@ struct mgf_solver {
     int device = 0;
     hipStream_t st = nullptr;
     int isolve = 0, nswpm = 0, nswpp = 0;
+    int smoother = 0;                     /* isolve, or 3: the lexicographic variant */
     void *arena = nullptr;
     size_t arena_bytes = 0;
     Program full;                         /* every instance */
@@ -520,6 +521,26 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
                            "batched problems must share isolve, nswpm and nswpp");
         }
     }
+    s->smoother = s->isolve;
+    if (o.gs_order != 0 && o.gs_order != 1) {
+        mgf_destroy(s);
+        return set_err(NULL, MGF_ERR_ARG, "gs_order must be 0 (red-black) or 1 (lexicographic)");
+    }
+    if (o.gs_order == 1 && s->isolve == 2) {
+        if (o.precision != MGF_PREC_FP64) {
+            mgf_destroy(s);
+            return set_err(NULL, MGF_ERR_UNSUPPORTED,
+                           "the lexicographic Gauss-Seidel variant runs in FP64 only");
+        }
+        for (const Inst &I : s->inst)
+            if (I.plan->prob.nfy > LEX_MAX_ROWS) {
+                mgf_destroy(s);
+                return set_err(NULL, MGF_ERR_UNSUPPORTED,
+                               "the lexicographic Gauss-Seidel variant needs nfy <= %d",
+                               LEX_MAX_ROWS);
+            }
+        s->smoother = 3;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         mgf_destroy(s);
@@ -688,20 +709,20 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
     hipStream_t st = s->st;
     mgf_launch_fluxes(G, st);
     mgf_launch_asm_u(G, st);
-    mgf_launch_smooth(G, 0, s->isolve, s->nswpm, st);
+    mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
     mgf_launch_asm_v(G, st);
-    mgf_launch_smooth(G, 1, s->isolve, s->nswpm, st);
+    mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
     mgf_launch_asm_p(G, err, st);
-    mgf_launch_smooth(G, 2, s->isolve, s->nswpp, st);
+    mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
     mgf_launch_update(G, 0, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) {
         mgf_launch_outflow(G, st);
         mgf_launch_asm_sc(G, st);
-        mgf_launch_smooth(G, 3, s->isolve, s->nswpm, st);
+        mgf_launch_smooth(G, 3, s->smoother, s->nswpm, st);
     } else {
         mgf_launch_asm_sc(G, st);
-        mgf_launch_smooth(G, 3, s->isolve, s->nswpm, st);
+        mgf_launch_smooth(G, 3, s->smoother, s->nswpm, st);
         mgf_launch_outflow(G, st);
     }
     return 0;
@@ -1112,6 +1133,7 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     const DevGroup &G = s->full.groups[g];
     const int field = (kernel == MGF_KERNEL_SWEEP_PP || kernel == MGF_KERNEL_RELAX_PP) ? 2 : 0;
     const int isolve = s->isolve == 1 ? 1 : 2;
+    const int fp64_smoother = s->smoother == 3 ? 3 : isolve;
     const int even = reps + (reps & 1);    /* an even count ends without a copy-back */
     real_t *save = NULL, *phi = NULL;
     const size_t n = (size_t)L0.npx * L0.npy;
@@ -1144,7 +1166,7 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
         } else if (mixed) {
             mgf_launch_sweeps_mixed(G, field, isolve, nl * spl, cfg, s->st);
         } else {
-            mgf_launch_smooth(G, field, isolve, nl, s->st);
+            mgf_launch_smooth(G, field, fp64_smoother, nl, s->st);
         }
     };
     launch(2);                                            /* warm up */

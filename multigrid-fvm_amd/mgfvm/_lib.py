@@ -46,7 +46,8 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("device", ctypes.c_int), ("strict_limits", ctypes.c_int),
                 ("sweeps_per_launch", ctypes.c_int), ("sweep_threads", ctypes.c_int),
-                ("split_assembly", ctypes.c_int), ("compact", ctypes.c_int)]
+                ("split_assembly", ctypes.c_int), ("gs_order", ctypes.c_int),
+                ("compact", ctypes.c_int)]
 
 
 class ProblemStruct(ctypes.Structure):
@@ -168,18 +169,18 @@ class Solver:
     def __init__(self, inputdata: str, pattern: str, precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, strict_limits: bool = False,
                  sweeps_per_launch: int = 0, sweep_threads: int = 0,
-                 split_assembly: bool = False, inlet_u=None):
+                 split_assembly: bool = False, inlet_u=None, gs_lex: bool = False):
         L = lib()
         prob = read_problem(inputdata, pattern)
         try:
             with InletProfile(prob, inlet_u):
                 self._init_from(prob, precision, use_graph, device, strict_limits,
-                                sweeps_per_launch, sweep_threads, split_assembly)
+                                sweeps_per_launch, sweep_threads, split_assembly, gs_lex)
         finally:
             free_problem(prob)
 
     def _init_from(self, prob, precision, use_graph, device, strict_limits, sweeps_per_launch=0,
-                   sweep_threads=0, split_assembly=False):
+                   sweep_threads=0, split_assembly=False, gs_lex=False):
         L = lib()
         opt = Options()
         L.mgf_options_default(ctypes.byref(opt))
@@ -190,6 +191,7 @@ class Solver:
         opt.sweeps_per_launch = sweeps_per_launch
         opt.sweep_threads = sweep_threads
         opt.split_assembly = 1 if split_assembly else 0
+        opt.gs_order = 1 if gs_lex else 0
         h = ctypes.c_void_p()
         rc = L.mgf_create(prob, ctypes.byref(opt), ctypes.byref(h))
         if rc != 0:
@@ -282,7 +284,8 @@ class Batch:
 
     def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, sweeps_per_launch: int = 0,
-                 sweep_threads: int = 0, split_assembly: bool = False, compact: bool = True):
+                 sweep_threads: int = 0, split_assembly: bool = False, compact: bool = True,
+                 gs_lex: bool = False):
         L = lib()
         probs = [read_problem(a, b) for a, b in pairs]
         try:
@@ -296,6 +299,7 @@ class Batch:
             opt.sweep_threads = sweep_threads
             opt.split_assembly = 1 if split_assembly else 0
             opt.compact = 1 if compact else 0
+            opt.gs_order = 1 if gs_lex else 0
             h = ctypes.c_void_p()
             rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))
             if rc != 0:

@@ -84,6 +84,7 @@ struct orc_solver {
     double *hist;                    /* nvcycle entries */
     int hist_len;
     int verbose;
+    int gs_lex;                      /* isolve 2 sweeps in lexicographic order (variant) */
 };
 
 /* names of the double arrays, in a fixed order (used by get/set) */
@@ -729,12 +730,38 @@ static void rbgs(lvl_t *l, double *phi, int nx, int ny, int nsw)
                 }
 }
 
+/* Lexicographic Gauss-Seidel: NOT a routine of the reference.  The labelled
+ * variant of isolve 2 this build offers: solvergs's loops (V6:1308-1350)
+ * without the colour tests, i.e. one in-place pass per sweep with j outer
+ * and i inner, each point updated with solvergs's expression.  Pinned by the
+ * mathematics only (tests/test_oracle_math.py: fixed point = the dense
+ * solve; the last point's equation holds after a sweep). */
+static void lexgs(lvl_t *l, double *phi, int nx, int ny, int nsw)
+{
+    int npx = l->npx;
+    for (int it = 0; it < nsw; ++it)
+        for (int j = 1; j < ny; ++j)
+            for (int i = 1; i < nx; ++i) {
+                int ij = j * npx + i;
+                double sum_nb = ((l->aw[ij] * phi[ij - 1] + l->ae[ij] * phi[ij + 1]) +
+                                 l->as[ij] * phi[ij - npx]) + l->an[ij] * phi[ij + npx];
+                phi[ij] = (sum_nb + l->su[ij]) / l->ap[ij];
+            }
+}
+
 static void smooth(orc_solver *s, lvl_t *l, double *phi, int nx, int ny)
 {
     int nsw = (phi == l->pp) ? s->cfg.nswpp : s->cfg.nswpm;   /* V6:1238 */
     if (s->cfg.isolve == 1) jacobi(l, phi, nx, ny, nsw);
-    if (s->cfg.isolve == 2) rbgs(l, phi, nx, ny, nsw);
+    if (s->cfg.isolve == 2) {
+        if (s->gs_lex)
+            lexgs(l, phi, nx, ny, nsw);
+        else
+            rbgs(l, phi, nx, ny, nsw);
+    }
 }
+
+void orc_set_gs_lex(orc_solver *s, int lex) { s->gs_lex = lex ? 1 : 0; }
 
 /* V6:696-737 */
 static void solve_u(orc_solver *s, lvl_t *l)
@@ -1187,6 +1214,7 @@ int orc_call(orc_solver *s, const char *fn, int ig, double *ret)
     else if (!strcmp(fn, "jacobi_pp")) jacobi(l, l->pp, l->npx - 1, l->npy - 1, cf->nswpp);
     else if (!strcmp(fn, "rbgs_u")) rbgs(l, l->u, l->npx - 2, l->npy - 1, cf->nswpm);
     else if (!strcmp(fn, "rbgs_pp")) rbgs(l, l->pp, l->npx - 1, l->npy - 1, cf->nswpp);
+    else if (!strcmp(fn, "lexgs_pp")) lexgs(l, l->pp, l->npx - 1, l->npy - 1, cf->nswpp);
     else if (c && !strcmp(fn, "restv")) restv(l, c);
     else if (c && !strcmp(fn, "restru")) restru(l, c, cf);
     else if (c && !strcmp(fn, "restrv")) restrv(l, c, cf);

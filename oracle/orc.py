@@ -37,6 +37,7 @@ def lib():
         L.orc_create_from_files.argtypes = [cp, cp, ctypes.POINTER(vp)]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_verbose.argtypes = [vp, ip]
+        L.orc_set_gs_lex.argtypes = [vp, ip]
         L.orc_num_levels.argtypes = [vp]
         L.orc_level_dims.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]
         L.orc_get.argtypes = [vp, ip, cp, ctypes.POINTER(dp)]
@@ -60,13 +61,15 @@ def _dptr(a: np.ndarray):
 class Oracle:
     """CPU restatement of the reference solver on one InputData/pattern pair."""
 
-    def __init__(self, inputdata: str, pattern: str):
+    def __init__(self, inputdata: str, pattern: str, gs_lex: bool = False):
         L = lib()
         h = ctypes.c_void_p()
         rc = L.orc_create_from_files(inputdata.encode(), pattern.encode(), ctypes.byref(h))
         if rc != 0:
             raise RuntimeError(f"oracle create failed ({rc}): {L.orc_last_error().decode()}")
         self.h = h
+        if gs_lex:
+            L.orc_set_gs_lex(h, 1)
         self.ngrid = L.orc_num_levels(h)
 
     def close(self):

@@ -146,3 +146,69 @@ def test_coarse_obstacles_are_all_children_rule(tmp_path):
         c = o.get(ig, "tag") == 0
         want = f[1:-1:2, 1:-1:2] & f[2::2, 1:-1:2] & f[1:-1:2, 2::2] & f[2::2, 2::2]
         assert np.array_equal(c[1:-1, 1:-1], want), ig
+
+
+def _random_system(o, seed):
+    """A random diagonally dominant 5-point system on the 8x8 p' unknowns of
+    level 0 (zero boundary); returns (coefficients, ap, su)."""
+    rng = np.random.default_rng(seed)
+    shape = (10, 10)
+    co = {n: np.zeros(shape) for n in ("aw", "ae", "as", "an")}
+    for n in co:
+        co[n][1:-1, 1:-1] = rng.uniform(0.1, 1.0, (8, 8))
+    ap = sum(co.values()) + 0.5
+    su = np.zeros(shape)
+    su[1:-1, 1:-1] = rng.normal(size=(8, 8))
+    for n, a in co.items():
+        o.set(0, n, a)
+    o.set(0, "ap", ap)
+    o.set(0, "su", su)
+    return co, ap, su
+
+
+def test_lexicographic_gs_converges_to_linear_solve(tmp_path):
+    """The lexicographic Gauss-Seidel variant (not in the reference; pinned
+    here by the mathematics): many sweeps reach the dense solution."""
+    prob = Problem(nfx=8, nfy=8, ngrid=1, ig_obs=1, isolve=2, nswpp=200, obstacles=[])
+    pin, ppat = prob.write(str(tmp_path / "lx"))
+    o = orc.Oracle(pin, ppat, gs_lex=True)
+    co, ap, su = _random_system(o, 1)
+    o.set(0, "pp", np.zeros((10, 10)))
+    o.call("lexgs_pp", 0)
+    x = o.get(0, "pp")[1:-1, 1:-1]
+    A = np.zeros((64, 64))
+    b = su[1:-1, 1:-1].ravel()
+    for jj in range(8):
+        for ii in range(8):
+            k = jj * 8 + ii
+            A[k, k] = ap[jj + 1, ii + 1]
+            for n, dj, di in (("aw", 0, -1), ("ae", 0, 1), ("as", -1, 0), ("an", 1, 0)):
+                j2, i2 = jj + dj, ii + di
+                if 0 <= j2 < 8 and 0 <= i2 < 8:
+                    A[k, j2 * 8 + i2] = -co[n][jj + 1, ii + 1]
+    np.testing.assert_allclose(x.ravel(), np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    o.close()
+
+
+def test_lexicographic_gs_sweep_order(tmp_path):
+    """One lexicographic sweep from zero: the first point sees only its
+    source (x = su / ap), each point's equation holds with its east and
+    north neighbours at their old value (zero), and the last point's
+    equation holds exactly afterwards (no neighbour of it changes later in
+    the sweep).  A red-black sweep fails the second property."""
+    prob = Problem(nfx=8, nfy=8, ngrid=1, ig_obs=1, isolve=2, nswpp=1, obstacles=[])
+    pin, ppat = prob.write(str(tmp_path / "ls"))
+    o = orc.Oracle(pin, ppat, gs_lex=True)
+    co, ap, su = _random_system(o, 2)
+    o.set(0, "pp", np.zeros((10, 10)))
+    o.call("lexgs_pp", 0)
+    x = o.get(0, "pp")
+    assert x[1, 1] == su[1, 1] / ap[1, 1]
+    # west / south neighbours at their new value, east / north at zero
+    for j in range(1, 9):
+        for i in range(1, 9):
+            s = co["aw"][j, i] * x[j, i - 1] + co["as"][j, i] * x[j - 1, i]
+            assert abs(ap[j, i] * x[j, i] - (s + su[j, i])) <= 1e-12 * (1 + abs(su[j, i]))
+    r_last = ap[8, 8] * x[8, 8] - (co["aw"][8, 8] * x[8, 7] + co["as"][8, 8] * x[7, 8] + su[8, 8])
+    assert abs(r_last) < 1e-12
+    o.close()
