@@ -220,8 +220,9 @@ def main():
                     help="run only the dominant kernel and a 1 GiB copy (for counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(problems are independent and would shard by problem; see DESIGN.md)",
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the problems of a batch are independent and would be split across GPUs "
+              "by problem, each GPU solving its share as one batch)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -44,7 +44,7 @@ extern "C" {
 #define MGF_ERR_LIMIT       -6   /* a size limit of this implementation is exceeded */
 #define MGF_ERR_DEVICE      -7   /* HIP runtime error */
 #define MGF_ERR_NONFINITE   -8   /* non-finite residual (the reference does not detect it) */
-#define MGF_ERR_UNSUPPORTED -9   /* option not supported by this build */
+#define MGF_ERR_UNSUPPORTED -9   /* option or configuration this library does not support */
 
 /* Everything InputData and pattern.txt carry (V6:27-39 globals, grammar
  * README.md:7-78).  Arrays are owned by whoever built the struct; the
