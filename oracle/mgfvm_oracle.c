@@ -731,7 +731,7 @@ static void rbgs(lvl_t *l, double *phi, int nx, int ny, int nsw)
 }
 
 /* Lexicographic Gauss-Seidel: NOT a routine of the reference.  The labelled
- * variant of isolve 2 this build offers: solvergs's loops (V6:1308-1350)
+ * variant of isolve 2 the GPU library offers: solvergs's loops (V6:1308-1350)
  * without the colour tests, i.e. one in-place pass per sweep with j outer
  * and i inner, each point updated with solvergs's expression.  Pinned by the
  * mathematics only (tests/test_oracle_math.py: fixed point = the dense

@@ -38,7 +38,7 @@ int orc_create_from_files(const char *inputdata, const char *pattern, orc_solver
 void orc_destroy(orc_solver *s);
 void orc_set_verbose(orc_solver *s, int v);
 /* isolve 2 sweeps in lexicographic order instead of red-black (a variant
- * this build defines; the reference has only red-black) */
+ * the GPU library defines; the reference has only red-black) */
 void orc_set_gs_lex(orc_solver *s, int lex);
 int orc_num_levels(const orc_solver *s);
 int orc_level_dims(const orc_solver *s, int ig, int *npx, int *npy);

@@ -122,7 +122,7 @@ def test_symmetric_channel(tmp_path):
 def test_converged_state_is_a_vcycle_fixed_point(tmp_path, name):
     """FAS consistency: once converged, another V-cycle changes nothing
     beyond round-off.  For coarse_free (obstacles only down to level 1, the
-    rest defined by this build: coarsened obstacles, masked residuals and
+    rest defined by the library: coarsened obstacles, masked residuals and
     corrections) this is what pins the definition; without the residual mask
     the obstacle faces' penalty-balanced residual would drive corrections."""
     prob = cases.CASES[name].replace(nvcycle=3000, convergence=1e-11)
