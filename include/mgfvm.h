@@ -108,6 +108,12 @@ typedef struct mgf_options {
                                   ones only (1, default), instead of launching over all
                                   of them with the stopped ones returning at once (0).
                                   Result-neutral. */
+    int lanes;                 /* batched solves: the running problems are split into
+                                  this many groups ("lanes", balanced by cell count), each
+                                  replaying its own V-cycle on its own stream, so one
+                                  lane's latency-bound coarse levels and launch tails
+                                  overlap another lane's work (1-4; 0: automatic, 3).
+                                  Result-neutral. */
 } mgf_options;
 
 typedef struct mgf_solver mgf_solver;

@@ -165,7 +165,8 @@ void mgf_launch_restrict(const DevPair &R, int restv, hipStream_t st);
  * (R: the transfer pair of the same levels, members in the same order) */
 void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, hipStream_t st);
 void mgf_launch_prolong(const DevPair &R, hipStream_t st);
-void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st);
+/* the stopping test for the n instances listed in members (device array) */
+void mgf_launch_end_cycle(DevCtl *ctls, const int *members, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */
 void mgf_launch_asm_mixed(const DevGroup &G, int field, int err, hipStream_t st);
 /* shape of the temporally blocked red-black sweeps: sweeps per launch,

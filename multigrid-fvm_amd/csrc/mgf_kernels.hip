@@ -833,11 +833,11 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, cons
 /* ---------------------------------------------------------------------- */
 /* end of a V-cycle: stopping test of V6:247-248                            */
 /* ---------------------------------------------------------------------- */
-__global__ void k_end_cycle(DevCtl *ctls, int n)
+__global__ void k_end_cycle(DevCtl *ctls, const int *members, int n)
 {
-    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (b >= n) return;
-    DevCtl *ctl = &ctls[b];
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    DevCtl *ctl = &ctls[members[k]];
     if (ctl->done) return;
     const double h = ctl->hist[ctl->cyc];
     if (!(h == h) || h == __builtin_huge_val() || h == -__builtin_huge_val()) ctl->nonfinite = 1;
@@ -1043,9 +1043,10 @@ void mgf_launch_prolong(const DevPair &R, hipStream_t st)
                        st, R.f, R.c);
 }
 
-void mgf_launch_end_cycle(DevCtl *ctls, int n, hipStream_t st)
+void mgf_launch_end_cycle(DevCtl *ctls, const int *members, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_end_cycle, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, ctls, n);
+    hipLaunchKernelGGL(k_end_cycle, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, ctls,
+                       members, n);
 }
 
 void mgf_launch_fill(real_t *a, double v, size_t n, hipStream_t st)

@@ -37,6 +37,7 @@ void mgf_options_default(mgf_options *o)
     o->split_assembly = 0;
     o->gs_order = 0;
     o->compact = 1;
+    o->lanes = 0;
 }
 
 static int err(int code, const char *fmt, const char *what)

@@ -26,6 +26,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "mgf_dev.h"
@@ -53,6 +54,7 @@ struct Op {
  * graph */
 struct Program {
     std::vector<int> members;             /* instance indices */
+    int *d_members = nullptr;             /* the same on the device (stopping test) */
     DevLevel *d_desc = nullptr;           /* all group descriptor arrays */
     std::vector<DevGroup> groups;
     std::vector<DevPair> pairs;
@@ -73,7 +75,10 @@ struct mgf_solver {
     void *arena = nullptr;
     size_t arena_bytes = 0;
     Program full;                         /* every instance */
-    Program live;                         /* the instances still running (compaction) */
+    std::vector<Program> lanes;           /* the running instances, one program per lane */
+    std::vector<hipStream_t> lane_st;     /* lane l's stream (lane 0: st) */
+    std::vector<hipEvent_t> lane_ev;      /* lane l's end of chunk, waited for on st */
+    hipStream_t est = nullptr;            /* the stream the enqueue_* calls launch on */
     /* single-instance step API: group of level ig, pair with fine level ig */
     std::vector<int> level_group;
     std::vector<int> level_pair;
@@ -383,6 +388,9 @@ static int build_program(mgf_solver *s, Program &P, const std::vector<int> &memb
     for (auto &g : gm) ndesc += g.size();
     for (auto &p : pf) ndesc += 2 * p.size();
     P.members = members;
+    HIPCHK(s, hipMalloc(&P.d_members, (members.size() + 1) * sizeof(int)));
+    HIPCHK(s, hipMemcpy(P.d_members, members.data(), members.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
     HIPCHK(s, hipMalloc(&P.d_desc, ndesc * sizeof(DevLevel)));
     std::vector<DevLevel> host;
     host.reserve(ndesc);
@@ -556,6 +564,9 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
         mgf_destroy(s);
         return set_err(NULL, MGF_ERR_DEVICE, "stream creation failed");
     }
+    s->est = s->st;
+    s->lane_st.assign(1, s->st);
+    s->lane_ev.assign(1, nullptr);
     size_t total = 0;
     for (Inst &I : s->inst)
         for (int ig = 0; ig < I.ng; ++ig)
@@ -613,15 +624,20 @@ static void release_program(Program &P)
 {
     if (P.graph_ready) (void)hipGraphExecDestroy(P.gexec);
     if (P.d_desc) (void)hipFree(P.d_desc);
+    if (P.d_members) (void)hipFree(P.d_members);
     P = Program();
 }
 
 extern "C" void mgf_destroy(mgf_solver *s)
 {
     if (!s) return;
-    if (s->st) (void)hipStreamSynchronize(s->st);
-    release_program(s->live);
+    for (hipStream_t l : s->lane_st)
+        if (l) (void)hipStreamSynchronize(l);
+    for (Program &P : s->lanes) release_program(P);
     release_program(s->full);
+    for (size_t l = 1; l < s->lane_st.size(); ++l) (void)hipStreamDestroy(s->lane_st[l]);
+    for (hipEvent_t e : s->lane_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->arena) (void)hipFree(s->arena);
     if (s->d_hist) (void)hipFree(s->d_hist);
     if (s->d_ctl) (void)hipFree(s->d_ctl);
@@ -670,7 +686,7 @@ extern "C" int mgf_level_dims(const mgf_solver *s, int ig, int *npx, int *npy)
 static int enqueue_relax_mixed(mgf_solver *s, const DevGroup &G, int leg, int err,
                                const DevPair *R)
 {
-    hipStream_t st = s->st;
+    hipStream_t st = s->est;
     const SweepCfg cfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads,
                                        s->opt.split_assembly);
     /* u, v and sc in one launch where the configuration allows: sc's
@@ -706,7 +722,7 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
                          const DevPair *R = nullptr)
 {
     if (s->opt.precision == MGF_PREC_MIXED) return enqueue_relax_mixed(s, G, leg, err, R);
-    hipStream_t st = s->st;
+    hipStream_t st = s->est;
     mgf_launch_fluxes(G, st);
     mgf_launch_asm_u(G, st);
     mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
@@ -742,8 +758,8 @@ static void enqueue_vcycle(mgf_solver *s, const Program &P)
             break;
         }
         case OP_RELAX_UP: enqueue_relax(s, P.groups[o.g], 1, 0); break;
-        case OP_RESTRICT: mgf_launch_restrict(P.pairs[o.g], !restv_done[o.g], s->st); break;
-        case OP_PROLONG: mgf_launch_prolong(P.pairs[o.g], s->st); break;
+        case OP_RESTRICT: mgf_launch_restrict(P.pairs[o.g], !restv_done[o.g], s->est); break;
+        case OP_PROLONG: mgf_launch_prolong(P.pairs[o.g], s->est); break;
         }
     }
 }
@@ -879,8 +895,9 @@ static int capture_graph(mgf_solver *s, Program &P)
     if (P.graph_ready) return MGF_OK;
     hipGraph_t g;
     HIPCHK(s, hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    s->est = s->st;
     enqueue_vcycle(s, P);
-    mgf_launch_end_cycle(s->d_ctl, s->nb, s->st);
+    mgf_launch_end_cycle(s->d_ctl, P.d_members, (int)P.members.size(), s->st);
     hipError_t e = hipStreamEndCapture(s->st, &g);
     if (e != hipSuccess)
         return set_err(s, MGF_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
@@ -892,19 +909,84 @@ static int capture_graph(mgf_solver *s, Program &P)
     return MGF_OK;
 }
 
+/* the lane count for n running instances: opt.lanes (0: MGF_AUTO_LANES),
+ * at most MGF_MAX_LANES and n.  Each lane's stream takes a hardware queue
+ * while it runs; HIP gives a process 4 by default, and with 4 lanes plus
+ * the caller's own stream two lanes end up sharing one (4 lanes measured
+ * slower than 1 on the batch benchmark, 3 fastest). */
+#define MGF_MAX_LANES 4
+#define MGF_AUTO_LANES 3
+static int lane_count(const mgf_solver *s, size_t n)
+{
+    int nl = s->opt.lanes > 0 ? s->opt.lanes : MGF_AUTO_LANES;
+    if (nl > MGF_MAX_LANES) nl = MGF_MAX_LANES;
+    if ((size_t)nl > n) nl = (int)n;
+    return nl < 1 ? 1 : nl;
+}
+
+/* streams and events for nl lanes (lane 0 is the solver's stream) */
+static int ensure_lanes(mgf_solver *s, int nl)
+{
+    while ((int)s->lane_st.size() < nl) {
+        hipStream_t st;
+        HIPCHK(s, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        s->lane_st.push_back(st);
+        s->lane_ev.push_back(nullptr);
+    }
+    for (int l = 0; l < nl; ++l)
+        if (!s->lane_ev[l]) HIPCHK(s, hipEventCreateWithFlags(&s->lane_ev[l], hipEventDisableTiming));
+    return MGF_OK;
+}
+
+/* the running instances split into nl programs of about equal cell count
+ * (largest first, each to the lightest lane), captured when graphs are on */
+static int build_lanes(mgf_solver *s, const std::vector<int> &alive, int nl)
+{
+    for (Program &P : s->lanes) release_program(P);
+    s->lanes.clear();
+    std::vector<int> order(alive);
+    auto cells = [&](int b) { return (double)s->inst[b].lv[0].npx * s->inst[b].lv[0].npy; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    std::vector<std::vector<int>> part(nl);
+    std::vector<double> load(nl, 0.0);
+    for (int b : order) {
+        int best = 0;
+        for (int l = 1; l < nl; ++l)
+            if (load[l] < load[best]) best = l;
+        part[best].push_back(b);
+        load[best] += cells(b);
+    }
+    int rc;
+    s->lanes.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        std::sort(part[l].begin(), part[l].end());
+        if ((rc = build_program(s, s->lanes[l], part[l]))) return rc;
+        if (s->opt.use_graph && (rc = capture_graph(s, s->lanes[l]))) return rc;
+    }
+    return MGF_OK;
+}
+
 /* Run V-cycles until every instance is done: its recorded residual fell
  * below its tolerance, or it ran its cycle limit (both decided on the
  * device).  The host enqueues cycles in growing chunks and polls the control
  * blocks between chunks; kernels of finished instances return at once, so
  * cycles enqueued past an instance's end change nothing.
  *
+ * Lanes: a single problem's V-cycle is a chain of dependent launches, many
+ * of them small (coarse levels) or in their tails, but different problems
+ * are independent.  A batch's running problems are split into lanes of
+ * about equal cell count, each replaying its own V-cycle graph on its own
+ * stream, so the GPU overlaps one lane's latency-bound phases with the
+ * others' work.
+ *
  * Compaction: launches over stopped instances still cost their dispatch
  * (and a group whose members have all stopped, its launch latency), which
  * in a batch whose slowest members run far longer than the rest is a
  * sizeable share of the tail.  When at most 3/4 of the instances of the
- * running schedule are still live, a schedule over the live ones alone is
- * built and captured, and replaces it from the next chunk on.  Every
- * instance's arithmetic is its own, so the results do not change. */
+ * running schedule are still live, the lanes are rebuilt over the live ones
+ * alone and replace the old ones from the next chunk on.
+ *
+ * Every instance's arithmetic is its own, so neither changes the results. */
 static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
 {
     int maxnv = 0;
@@ -913,36 +995,65 @@ static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
     if (rc) return rc;
     if ((rc = reset_ctl(s, 0, tol, limit))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
-    release_program(s->live);
-    Program *P = &s->full;
-    if (s->opt.use_graph && (rc = capture_graph(s, *P))) return rc;
+    for (Program &P : s->lanes) release_program(P);
+    s->lanes.clear();
+    std::vector<int> alive;
+    for (int b = 0; b < s->nb; ++b)
+        if (!s->h_ctl[b].done) alive.push_back(b);
+    int nl = lane_count(s, alive.size());
+    /* the running programs: the full one while it is the only lane */
+    std::vector<Program *> run;
+    if (nl <= 1) {
+        if (s->opt.use_graph && (rc = capture_graph(s, s->full))) return rc;
+        run.push_back(&s->full);
+    } else {
+        if ((rc = ensure_lanes(s, nl)) || (rc = build_lanes(s, alive, nl))) return rc;
+        for (Program &P : s->lanes) run.push_back(&P);
+    }
+    size_t scheduled = s->full.members.size();
+    if (nl > 1) scheduled = alive.size();
     int launched = 0, chunk = 1;
     while (launched < maxnv) {
         const int k = chunk < maxnv - launched ? chunk : maxnv - launched;
-        for (int t = 0; t < k; ++t) {
-            if (s->opt.use_graph) {
-                HIPCHK(s, hipGraphLaunch(P->gexec, s->st));
-            } else {
-                enqueue_vcycle(s, *P);
-                mgf_launch_end_cycle(s->d_ctl, s->nb, s->st);
+        const int nr = (int)run.size();
+        /* the other lanes start after what the solver's stream holds */
+        if (nr > 1) {
+            HIPCHK(s, hipEventRecord(s->lane_ev[0], s->st));
+            for (int l = 1; l < nr; ++l)
+                HIPCHK(s, hipStreamWaitEvent(s->lane_st[l], s->lane_ev[0], 0));
+        }
+        for (int t = 0; t < k; ++t)
+            for (int l = 0; l < nr; ++l) {
+                Program &P = *run[l];
+                hipStream_t st = s->lane_st[l];
+                if (s->opt.use_graph) {
+                    HIPCHK(s, hipGraphLaunch(P.gexec, st));
+                } else {
+                    s->est = st;
+                    enqueue_vcycle(s, P);
+                    mgf_launch_end_cycle(s->d_ctl, P.d_members, (int)P.members.size(), st);
+                    s->est = s->st;
+                }
             }
+        for (int l = 1; l < nr; ++l) {
+            HIPCHK(s, hipEventRecord(s->lane_ev[l], s->lane_st[l]));
+            HIPCHK(s, hipStreamWaitEvent(s->st, s->lane_ev[l], 0));
         }
         if ((rc = check_launch(s))) return rc;
         launched += k;
-        if ((rc = read_ctl(s))) return rc;       /* the stream is idle from here */
-        std::vector<int> alive;
-        for (int b : P->members)
-            if (!s->h_ctl[b].done) alive.push_back(b);
+        if ((rc = read_ctl(s))) return rc;       /* every lane is idle from here */
+        alive.clear();
+        for (const Program *P : run)
+            for (int b : P->members)
+                if (!s->h_ctl[b].done) alive.push_back(b);
         if (alive.empty()) break;
-        if (s->opt.use_graph && s->opt.compact && 4 * alive.size() <= 3 * P->members.size()) {
-            Program np;
-            if ((rc = build_program(s, np, alive)) || (rc = capture_graph(s, np))) {
-                release_program(np);
-                return rc;
-            }
-            release_program(s->live);
-            s->live = np;
-            P = &s->live;
+        if (s->opt.use_graph && s->opt.compact && 4 * alive.size() <= 3 * scheduled) {
+            std::sort(alive.begin(), alive.end());
+            nl = lane_count(s, alive.size());
+            if ((rc = ensure_lanes(s, nl)) || (rc = build_lanes(s, alive, nl))) return rc;
+            run.clear();
+            for (Program &P : s->lanes) run.push_back(&P);
+            scheduled = alive.size();
         }
         if (chunk < 32) chunk *= 2;
     }

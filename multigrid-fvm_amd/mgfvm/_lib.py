@@ -47,7 +47,7 @@ class Options(ctypes.Structure):
                 ("device", ctypes.c_int), ("strict_limits", ctypes.c_int),
                 ("sweeps_per_launch", ctypes.c_int), ("sweep_threads", ctypes.c_int),
                 ("split_assembly", ctypes.c_int), ("gs_order", ctypes.c_int),
-                ("compact", ctypes.c_int)]
+                ("compact", ctypes.c_int), ("lanes", ctypes.c_int)]
 
 
 class ProblemStruct(ctypes.Structure):
@@ -285,7 +285,7 @@ class Batch:
     def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, sweeps_per_launch: int = 0,
                  sweep_threads: int = 0, split_assembly: bool = False, compact: bool = True,
-                 gs_lex: bool = False):
+                 gs_lex: bool = False, lanes: int = 0):
         L = lib()
         probs = [read_problem(a, b) for a, b in pairs]
         try:
@@ -300,6 +300,7 @@ class Batch:
             opt.split_assembly = 1 if split_assembly else 0
             opt.compact = 1 if compact else 0
             opt.gs_order = 1 if gs_lex else 0
+            opt.lanes = lanes
             h = ctypes.c_void_p()
             rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))
             if rc != 0:

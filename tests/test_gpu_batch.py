@@ -198,3 +198,33 @@ def test_compaction_result_neutral(tmp_path, precision):
                                           bt.get(k, ig, name).view(np.uint64)), (rep, k, ig, name)
     ref.close()
     bt.close()
+
+
+@pytest.mark.parametrize("precision,lanes,graph", [("mixed", 2, True), ("mixed", 3, True),
+                                                   ("mixed", 4, False), ("fp64", 2, True)])
+def test_lanes_result_neutral(tmp_path, precision, lanes, graph):
+    """Splitting the running problems into lanes (one V-cycle graph per lane,
+    each on its own stream, rebuilt as problems stop) gives every problem
+    the history and fields of a single-lane batch, bit for bit."""
+    import mgfvm
+    prec = mgfvm.PREC_FP64 if precision == "fp64" else mgfvm.PREC_MIXED
+    base = _mixed_problems()
+    limits = [3, 7, 12, 20, 33, 50, 70]
+    probs = [p.replace(nvcycle=lim, convergence=1e-14) for p, lim in zip(base, limits)]
+    pairs = [p.write(str(tmp_path / f"l{k}")) for k, p in enumerate(probs)]
+    _say(f"[gpu] lanes {lanes} {precision} graph={graph}")
+    ref = mgfvm.Batch(pairs, precision=prec, lanes=1)
+    nr, cr = ref.solve()
+    bt = mgfvm.Batch(pairs, precision=prec, lanes=lanes, use_graph=graph)
+    for rep in range(2):
+        if rep:
+            bt.reset()
+        nc, cv = bt.solve()
+        assert (list(nc), list(cv)) == (list(nr), list(cr)), rep
+        for k in range(len(probs)):
+            assert np.array_equal(ref.history(k), bt.history(k)), (rep, k)
+            for name in ("u", "v", "p", "pp", "sc"):
+                assert np.array_equal(ref.get(k, 0, name).view(np.uint64),
+                                      bt.get(k, 0, name).view(np.uint64)), (rep, k, name)
+    ref.close()
+    bt.close()
