@@ -185,7 +185,9 @@ struct SweepCfg {
 /* sweeps of a launch that also assembles the equation */
 #define MGF_MAX_FUSED_SWEEPS 3
 #define MGF_AUTO_SWEEP_THREADS 512
-#define MGF_AUTO_ASM_THREADS 256
+#ifndef MGF_AUTO_ASM_THREADS
+#define MGF_AUTO_ASM_THREADS 512
+#endif
 SweepCfg mgf_sweep_cfg(int sweeps_per_launch, int threads, int split_assembly);
 /* the first (assembling) launch of a fused relaxation alone, for timing;
  * returns compulsory bytes per unknown */

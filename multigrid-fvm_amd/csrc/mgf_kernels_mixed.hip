@@ -195,16 +195,21 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #define MODE_ASM 1
 #define MODE_ASM_STORE 2
 #define MODE_ASM_INL 3
-template <int S>
+/* tile rows of the merged u, v, sc launch: taller than the other launches'
+ * tiles, as its halo is assembled three times over (once per field) */
+#ifndef MGF_UVS_TILE_ROWS
+#define MGF_UVS_TILE_ROWS 32
+#endif
+template <int S, int TYR = MGF_SWEEP_TILE_ROWS>
 struct RbShape {
-    static constexpr int TY = MGF_SWEEP_TILE_ROWS;
+    static constexpr int TY = TYR;
     static constexpr int H = 2 * S, RW = BX + 2 * H, RH = TY + 2 * H, NR = RW * RH;
 };
 
 /* the work of one workgroup on tile T of level L; t: NR floats of LDS, mg
  * and rows: the |div| rows of an assembling p' launch (TY x BX and TY
  * doubles; unused otherwise) */
-template <int S, int NT, int MODE, int FIELD>
+template <int S, int NT, int MODE, int FIELD, int TYR = MGF_SWEEP_TILE_ROWS>
 __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, int field_rt,
                                            int launch, int last, float *t, double *mg,
                                            double *rows)
@@ -212,9 +217,10 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     /* FIELD >= 0: the field is fixed at compile time (assembling launches);
      * -1: taken from field_rt */
     const int field = FIELD >= 0 ? FIELD : field_rt;
-    constexpr int TY = RbShape<S>::TY;
-    constexpr int H = RbShape<S>::H, RW = RbShape<S>::RW, RH = RbShape<S>::RH;
-    constexpr int NR = RbShape<S>::NR;
+    using Sh = RbShape<S, TYR>;
+    constexpr int TY = Sh::TY;
+    constexpr int H = Sh::H, RW = Sh::RW, RH = Sh::RH;
+    constexpr int NR = Sh::NR;
     constexpr int NPT = (NR + NT - 1) / NT, NW = NT / BX;
     constexpr bool ASM = MODE != MODE_LOAD;
     constexpr bool INL = MODE == MODE_ASM_INL;
@@ -367,15 +373,15 @@ __global__ void __launch_bounds__(NT) k_rb32x_uvs(const DevLevel *Ls)
     const int f = T3.z - 3 * T.z;
     const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[T.z];
     if (L.ctl->done) return;
-    __shared__ float t[RbShape<S>::NR];
+    __shared__ float t[RbShape<S, MGF_UVS_TILE_ROWS>::NR];
     __shared__ double mg[BX];
     __shared__ double rows[1];
     if (f == 0)
-        rb32x_tile<S, NT, MODE_ASM_INL, 0>(L, T, 0, 0, 1, t, mg, rows);
+        rb32x_tile<S, NT, MODE_ASM_INL, 0, MGF_UVS_TILE_ROWS>(L, T, 0, 0, 1, t, mg, rows);
     else if (f == 1)
-        rb32x_tile<S, NT, MODE_ASM_INL, 1>(L, T, 1, 0, 1, t, mg, rows);
+        rb32x_tile<S, NT, MODE_ASM_INL, 1, MGF_UVS_TILE_ROWS>(L, T, 1, 0, 1, t, mg, rows);
     else
-        rb32x_tile<S, NT, MODE_ASM_INL, 3>(L, T, 3, 0, 1, t, mg, rows);
+        rb32x_tile<S, NT, MODE_ASM_INL, 3, MGF_UVS_TILE_ROWS>(L, T, 3, 0, 1, t, mg, rows);
 }
 
 /* one FP32 Jacobi sweep of A d = r (wj = 1, V6:1241) */
@@ -636,7 +642,7 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
 template <int S>
 static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
 {
-    const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_SWEEP_TILE_ROWS);
+    const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_UVS_TILE_ROWS);
     const dim3 g(g0.x, g0.y, 3 * g0.z);
     if (nt == 1024)
         hipLaunchKernelGGL((k_rb32x_uvs<S, 1024>), g, dim3(BX, 16), 0, st, G.d);
