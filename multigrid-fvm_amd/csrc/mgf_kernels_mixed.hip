@@ -200,6 +200,10 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #ifndef MGF_UVS_TILE_ROWS
 #define MGF_UVS_TILE_ROWS 32
 #endif
+/* tile rows of the other assembling launches */
+#ifndef MGF_ASM_TILE_ROWS
+#define MGF_ASM_TILE_ROWS MGF_SWEEP_TILE_ROWS
+#endif
 template <int S, int TYR = MGF_SWEEP_TILE_ROWS>
 struct RbShape {
     static constexpr int TY = TYR;
@@ -346,16 +350,16 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     }
 }
 
-template <int S, int NT, int MODE, int FIELD>
+template <int S, int NT, int MODE, int FIELD, int TYR = MGF_SWEEP_TILE_ROWS>
 __global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, int launch,
                                               int last)
 {
     LEVEL(Ls);
-    constexpr int MR = MODE != MODE_LOAD ? RbShape<S>::TY : 1;
-    __shared__ float t[RbShape<S>::NR];
+    constexpr int MR = MODE != MODE_LOAD ? TYR : 1;
+    __shared__ float t[RbShape<S, TYR>::NR];
     __shared__ double mg[MR * BX];
     __shared__ double rows[MR];
-    rb32x_tile<S, NT, MODE, FIELD>(L, T, field_rt, launch, last, t, mg, rows);
+    rb32x_tile<S, NT, MODE, FIELD, TYR>(L, T, field_rt, launch, last, t, mg, rows);
 }
 
 /* The u, v and sc relaxations of a group in one launch (S sweeps each, the
@@ -450,16 +454,19 @@ template <int S, int MODE, int FIELD>
 static void launch_rbx_f(const DevGroup &G, int field, int launch, int last, int nx, int ny,
                          int nt, hipStream_t st)
 {
-    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SWEEP_TILE_ROWS);
+    /* assembling launches may take taller tiles (their halo costs an
+     * assembly per point); the launches on the stored equation keep theirs */
+    constexpr int TYR = MODE == MODE_LOAD ? MGF_SWEEP_TILE_ROWS : MGF_ASM_TILE_ROWS;
+    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, TYR);
     if (nt == 1024)
-        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE, FIELD>), g, dim3(BX, 16), 0, st, G.d, field,
-                           launch, last);
+        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE, FIELD, TYR>), g, dim3(BX, 16), 0, st, G.d,
+                           field, launch, last);
     else if (nt == 512)
-        hipLaunchKernelGGL((k_rb32x<S, 512, MODE, FIELD>), g, dim3(BX, 8), 0, st, G.d, field,
-                           launch, last);
+        hipLaunchKernelGGL((k_rb32x<S, 512, MODE, FIELD, TYR>), g, dim3(BX, 8), 0, st, G.d,
+                           field, launch, last);
     else
-        hipLaunchKernelGGL((k_rb32x<S, 256, MODE, FIELD>), g, dim3(BX, 4), 0, st, G.d, field,
-                           launch, last);
+        hipLaunchKernelGGL((k_rb32x<S, 256, MODE, FIELD, TYR>), g, dim3(BX, 4), 0, st, G.d,
+                           field, launch, last);
 }
 
 /* sweeps on the stored equation (any field) */
