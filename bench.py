@@ -134,7 +134,8 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
             "mean_us": round(us, 2), "bytes_per_launch": nbytes,
             "bytes_per_cell": round(nbytes / ncell, 2),
             "sweep_kernel": {"kernel": "k_rb32x<3,512,0,-1>: 3 FP32 red-black p' sweeps on "
-                                       "the stored equation, same members",
+                                       "a stored equation (split assembly, or p' beyond six "
+                                       "sweeps), same members",
                              "mean_us": round(us_s, 2), "bytes_per_launch": nbytes_s,
                              "effective_GBps": round(nbytes_s / (us_s * 1e-6) / 1e9, 1)}}
 

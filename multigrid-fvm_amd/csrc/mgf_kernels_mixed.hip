@@ -204,6 +204,17 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #ifndef MGF_ASM_TILE_ROWS
 #define MGF_ASM_TILE_ROWS MGF_SWEEP_TILE_ROWS
 #endif
+/* p' relaxations of up to this many sweeps run as one assembling launch on
+ * tiles of MGF_PP_TILE_ROWS rows (0: the assemble-and-store launch plus
+ * launches on the stored equation).  Six sweeps in one launch assemble a
+ * 12-point halo (2.4x the tile on 32 rows) but never store and reload the
+ * FP32 equation; measured 2% faster on the batch than 3 + 3. */
+#ifndef MGF_PP_ONE_LAUNCH
+#define MGF_PP_ONE_LAUNCH 6
+#endif
+#ifndef MGF_PP_TILE_ROWS
+#define MGF_PP_TILE_ROWS 32
+#endif
 template <int S, int TYR = MGF_SWEEP_TILE_ROWS>
 struct RbShape {
     static constexpr int TY = TYR;
@@ -587,11 +598,33 @@ static void launch_small_s(const DevGroup &G, int field, int last, int nx, int n
  * assembles); lastbits: the `last` argument of the final launch (bit 0:
  * final; bit 1, p' only: also stash the updated p of copy origins in tmp).
  * Returns the number of launches. */
+template <int S>
+static void launch_pp_one_s(const DevGroup &G, int last, int nx, int ny, int nt, hipStream_t st)
+{
+    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_PP_TILE_ROWS);
+    if (nt == 1024)
+        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM, 2, MGF_PP_TILE_ROWS>), g, dim3(BX, 16), 0,
+                           st, G.d, 2, 0, last);
+    else
+        hipLaunchKernelGGL((k_rb32x<S, 512, MODE_ASM, 2, MGF_PP_TILE_ROWS>), g, dim3(BX, 8), 0,
+                           st, G.d, 2, 0, last);
+}
+
 static int launch_fused(const DevGroup &G, int field, int nsw, SweepCfg cfg, int lastbits,
                         hipStream_t st)
 {
     int nx, ny;
     sweep_extents(G, field, nx, ny);
+    if (field == 2 && nsw <= MGF_PP_ONE_LAUNCH && nsw > MGF_MAX_FUSED_SWEEPS &&
+        !small_group(G)) {
+        const int nt = cfg.nt_asm == 1024 ? 1024 : 512;
+        switch (nsw) {
+        case 4: launch_pp_one_s<4>(G, lastbits, nx, ny, nt, st); break;
+        case 5: launch_pp_one_s<5>(G, lastbits, nx, ny, nt, st); break;
+        default: launch_pp_one_s<6>(G, lastbits, nx, ny, nt, st); break;
+        }
+        return 1;
+    }
     if (small_group(G) && nsw <= MGF_MAX_SWEEPS_PER_LAUNCH) {
         switch (nsw) {
         case 1: launch_small_s<1>(G, field, lastbits, nx, ny, st); break;
@@ -700,11 +733,17 @@ double mgf_launch_relax_first_mixed(const DevGroup &G, int field, int nsw, Sweep
     int nx, ny;
     sweep_extents(G, field, nx, ny);
     int lim = cfg.spl < MGF_MAX_FUSED_SWEEPS ? cfg.spl : MGF_MAX_FUSED_SWEEPS;
-    const int n = nsw < lim ? nsw : lim;
+    const bool one = field == 2 && nsw <= MGF_PP_ONE_LAUNCH && nsw > MGF_MAX_FUSED_SWEEPS &&
+                     !small_group(G);
+    const int n = one ? nsw : (nsw < lim ? nsw : lim);
     const int last = n == nsw;
     *n_out = n;
     if (n < 1 || nx - 1 < 1 || ny - 1 < 1) return 0.0;
-    launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
+    /* as launch_fused issues it (p' written to pp, not to the stash) */
+    if (one)
+        (void)launch_fused(G, field, nsw, cfg, 1, st);
+    else
+        launch_asm(G, field, last, !last, n, nx, ny, cfg.nt_asm, st);
     double b = 1.0 + (last ? 0.0 : 24.0);
     switch (field) {
     case 0:
