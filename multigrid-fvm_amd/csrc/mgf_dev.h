@@ -90,6 +90,9 @@ struct DevPhys {
      * (when u, v and sc relax in one launch) that of sc, applied there too */        \
     GQ float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32, *du32, *dv32;    \
     GQ float *ds32;                                                                   \
+    /* the merged relaxation's u and v with their corrections applied (FP64, over    \
+     * the swept ranges), read by its p' assembly and update instead of u + du */     \
+    GQ real_t *uc, *vc;                                                               \
     /* obstacle data */                                                               \
     GQ const uint8_t *flags;                                                          \
     GQ const uint64_t *sccode;                                                        \

@@ -340,7 +340,8 @@ __global__ void k_pstash(const DevLevel *Ls)
 /* PENDING >= 1: mixed precision, where the corrections of the u and v
  * solves are still in du32 / dv32 over their swept ranges (the value of u
  * there is u + du, formed exactly as the solve's own update would); 2: the
- * sc solve ran in the same launch as u and v, and its correction in ds32 is
+ * sc solve ran in the same launch as u and v (which left u + du, v + dv in
+ * uc, vc), and its correction in ds32 is
  * applied here (sc += d, as k_apply_mx would; the update does not read sc),
  * and the last p' launch left the updated p of every copy origin in tmp
  * (the value k_pstash would stash) */
@@ -356,11 +357,19 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     const int nx = L.npx;
     const int ij = j * nx + i;
     const real_t pp = L.pp[ij];
-    real_t u = L.u[ij], v = L.v[ij];
-    if (PENDING) {
-        const float du = L.du32[ij], dv = L.dv32[ij];
-        u = i <= nx - 3 ? u + (double)du : u;
-        v = j <= L.npy - 3 ? v + (double)dv : v;
+    real_t u, v;
+    if (PENDING == 2) {
+        /* the merged relaxation formed u + du, v + dv in uc, vc */
+        u = (i <= nx - 3 ? L.uc : L.u)[ij];
+        v = (j <= L.npy - 3 ? L.vc : L.v)[ij];
+    } else {
+        u = L.u[ij];
+        v = L.v[ij];
+        if (PENDING) {
+            const float du = L.du32[ij], dv = L.dv32[ij];
+            u = i <= nx - 3 ? u + (double)du : u;
+            v = j <= L.npy - 3 ? v + (double)dv : v;
+        }
     }
     Cell4 c;
     c.u = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];

@@ -98,21 +98,30 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int
 }
 
 /* masserror + coefp in FP64 (V6:1018-1125) at one cell, rounded to FP32;
- * mag = |div| */
+ * mag = |div|.  u and v carry their pending corrections over their swept
+ * ranges (u faces 1..npx-3, v faces 1..npy-3): the values are u + du,
+ * v + dv, formed as the solves' own update would form them -- here (CORR
+ * false), or already formed by the merged relaxation in uc, vc (CORR). */
+template <bool CORR = false>
 __device__ __forceinline__ Pt32 asm_pt_p(const DevLevelG &L, int i, int j, double &mag)
 {
     const DevPhys &P = L.P;
     const int nx = L.npx;
     const int ij = j * nx + i, ijs = ij - nx;
-    /* u and v carry their pending corrections over their swept ranges (u
-     * faces 1..npx-3, v faces 1..npy-3): the values are u + du, v + dv,
-     * formed as the solves' own update would form them */
-    const double u0 = L.u[ij], u1 = L.u[ij - 1], v0 = L.v[ij], v1 = L.v[ijs];
-    const float du0 = L.du32[ij], du1 = L.du32[ij - 1], dv0 = L.dv32[ij], dv1 = L.dv32[ijs];
-    const double ue = i <= nx - 3 ? u0 + (double)du0 : u0;
-    const double uw = i - 1 >= 1 ? u1 + (double)du1 : u1;
-    const double vn = j <= L.npy - 3 ? v0 + (double)dv0 : v0;
-    const double vs = j - 1 >= 1 ? v1 + (double)dv1 : v1;
+    double ue, uw, vn, vs;
+    if (CORR) {
+        ue = (i <= nx - 3 ? L.uc : L.u)[ij];
+        uw = (i - 1 >= 1 ? L.uc : L.u)[ij - 1];
+        vn = (j <= L.npy - 3 ? L.vc : L.v)[ij];
+        vs = (j - 1 >= 1 ? L.vc : L.v)[ijs];
+    } else {
+        const double u0 = L.u[ij], u1 = L.u[ij - 1], v0 = L.v[ij], v1 = L.v[ijs];
+        const float du0 = L.du32[ij], du1 = L.du32[ij - 1], dv0 = L.dv32[ij], dv1 = L.dv32[ijs];
+        ue = i <= nx - 3 ? u0 + (double)du0 : u0;
+        uw = i - 1 >= 1 ? u1 + (double)du1 : u1;
+        vn = j <= L.npy - 3 ? v0 + (double)dv0 : v0;
+        vs = j - 1 >= 1 ? v1 + (double)dv1 : v1;
+    }
     double div = ((L.ax * ue - L.ax * uw) + L.ay * vn) - L.ay * vs;
     double su = -div;
     mag = fabs(div);
@@ -195,6 +204,8 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #define MODE_ASM 1
 #define MODE_ASM_STORE 2
 #define MODE_ASM_INL 3
+/* MODE_ASM for the merged relaxation's p': u + du, v + dv read from uc, vc */
+#define MODE_ASM_CORR 4
 /* tile rows of the merged u, v, sc launch: taller than the other launches'
  * tiles, as its halo is assembled three times over (once per field) */
 #ifndef MGF_UVS_TILE_ROWS
@@ -239,6 +250,7 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     constexpr int NPT = (NR + NT - 1) / NT, NW = NT / BX;
     constexpr bool ASM = MODE != MODE_LOAD;
     constexpr bool INL = MODE == MODE_ASM_INL;
+    constexpr bool CORR = MODE == MODE_ASM_CORR;
     constexpr int MR = ASM ? TY : 1;
     int nx, ny;
     extents(L, field, nx, ny);
@@ -282,7 +294,7 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                 if (ASM) {
                     if (pfield) {
                         double mag;
-                        pt[k] = asm_pt_p(L, gi, gj, mag);
+                        pt[k] = asm_pt_p<CORR>(L, gi, gj, mag);
                         if (d >= H) mg[(r - H) * BX + (c - H)] = mag;
                     } else {
                         pt[k] = asm_pt<INL>(L, field, gi, gj, apk[ASM ? k : 0]);
@@ -330,6 +342,11 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                  * obstacle-edge copies (what k_pstash would stash) */
                 if (m & (1u << 10))
                     L.tmp[g] = L.p[g] + L.P.relxp * (ppv - 0.0);
+            } else if (INL && field == 0) {
+                /* the merged relaxation: u + du formed here, once */
+                L.uc[g] = L.u[g] + (double)t[p];
+            } else if (INL && field == 1) {
+                L.vc[g] = L.v[g] + (double)t[p];
             } else {
                 dst[g] = t[p];
             }
@@ -706,11 +723,39 @@ int mgf_launch_uvs_mixed(const DevGroup &G, int isolve, int nswm, SweepCfg cfg, 
     return 1;
 }
 
+/* the merged relaxation's p': one assembling launch with all S sweeps,
+ * reading u + du, v + dv from uc, vc, writing pp and the stash of the copy
+ * origins (last = 3) */
+template <int S>
+static void launch_pp_merged_s(const DevGroup &G, hipStream_t st)
+{
+    int nx, ny;
+    sweep_extents(G, 2, nx, ny);
+    if (small_group(G)) {
+        const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SWEEP_TILE_ROWS);
+        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM_CORR, 2>), g, dim3(BX, 16), 0, st, G.d, 2,
+                           0, 3);
+    } else {
+        const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_PP_TILE_ROWS);
+        hipLaunchKernelGGL((k_rb32x<S, 512, MODE_ASM_CORR, 2, MGF_PP_TILE_ROWS>), g, dim3(BX, 8),
+                           0, st, G.d, 2, 0, 3);
+    }
+}
+
 int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
                                   SweepCfg cfg, int err, const DevPair *R, hipStream_t st)
 {
-    if (nswp < 1 || !mgf_launch_uvs_mixed(G, isolve, nswm, cfg, st)) return 0;
-    (void)launch_fused(G, 2, nswp, cfg, 1 | 2, st);
+    if (nswp < 1 || nswp > MGF_MAX_SWEEPS_PER_LAUNCH ||
+        !mgf_launch_uvs_mixed(G, isolve, nswm, cfg, st))
+        return 0;
+    switch (nswp) {
+    case 1: launch_pp_merged_s<1>(G, st); break;
+    case 2: launch_pp_merged_s<2>(G, st); break;
+    case 3: launch_pp_merged_s<3>(G, st); break;
+    case 4: launch_pp_merged_s<4>(G, st); break;
+    case 5: launch_pp_merged_s<5>(G, st); break;
+    default: launch_pp_merged_s<6>(G, st); break;
+    }
     if (err) mgf_launch_reduce_err(G, err == 2, st);
     if (R && !G.any_pbound_x && !G.any_pbound_y) {
         mgf_launch_update_restv(G, *R, 2, st);

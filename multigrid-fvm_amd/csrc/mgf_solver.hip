@@ -130,7 +130,7 @@ static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
     size_t t = 27 * align256(n * sizeof(real_t));  /* 23 arrays + 4 initial-state copies */
-    if (mixed) t += 11 * align256(n * sizeof(float));
+    if (mixed) t += 11 * align256(n * sizeof(float)) + 2 * align256(n * sizeof(real_t));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
     t += align256(n * sizeof(uint64_t));            /* sccode */
@@ -256,6 +256,8 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
             float **a32[11] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
                                &L.da32, &L.db32, &L.du32, &L.dv32, &L.ds32};
             for (int k = 0; k < 11; ++k) *a32[k] = (float *)take(n * sizeof(float));
+            L.uc = (real_t *)take(nb);
+            L.vc = (real_t *)take(nb);
         }
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
