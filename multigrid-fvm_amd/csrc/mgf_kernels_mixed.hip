@@ -226,6 +226,10 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #ifndef MGF_PP_TILE_ROWS
 #define MGF_PP_TILE_ROWS 32
 #endif
+/* threads of the merged relaxation's p' launch (512 or 1024) */
+#ifndef MGF_PP_THREADS
+#define MGF_PP_THREADS 512
+#endif
 template <int S, int TYR = MGF_SWEEP_TILE_ROWS>
 struct RbShape {
     static constexpr int TY = TYR;
@@ -737,8 +741,8 @@ static void launch_pp_merged_s(const DevGroup &G, hipStream_t st)
                            0, 3);
     } else {
         const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_PP_TILE_ROWS);
-        hipLaunchKernelGGL((k_rb32x<S, 512, MODE_ASM_CORR, 2, MGF_PP_TILE_ROWS>), g, dim3(BX, 8),
-                           0, st, G.d, 2, 0, 3);
+        hipLaunchKernelGGL((k_rb32x<S, MGF_PP_THREADS, MODE_ASM_CORR, 2, MGF_PP_TILE_ROWS>), g,
+                           dim3(BX, MGF_PP_THREADS / BX), 0, st, G.d, 2, 0, 3);
     }
 }
 
