@@ -269,7 +269,8 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
  * apv, and the FP32 equation when later launches reuse it).
  * MGF_KERNEL_RELAX_UVS: bytes per interior cell are the union of what its
  * three equations must move: u, v, p, sc, resu, resv, ressc read (FP64),
- * the flag byte, the three FP32 corrections and apu, apv written. */
+ * the flag byte; u + du and v + dv (FP64), the sc correction (FP32), apu and
+ * apv written (93 bytes). */
 int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps,
                     double *mean_us, double *bytes_per_launch);
 

@@ -1306,7 +1306,9 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     if (uvs) {
         const double ncell = (double)(G.npx - 2) * (double)(G.npy - 2) * (double)G.count;
         if (mean_us) *mean_us = 1000.0 * (double)ms / (double)even;
-        if (bytes_per_launch) *bytes_per_launch = (7 * 8.0 + 1.0 + 3 * 4.0 + 2 * 8.0) * ncell;
+        /* u, v, p, sc, resu, resv, ressc and the flag byte read; u + du,
+         * v + dv (FP64), the sc correction (FP32), apu and apv written */
+        if (bytes_per_launch) *bytes_per_launch = (7 * 8.0 + 1.0 + 2 * 8.0 + 4.0 + 2 * 8.0) * ncell;
         return MGF_OK;
     }
     int nx = field == 2 ? G.npx - 1 : G.npx - 2, ny = G.npy - 1;
