@@ -226,6 +226,12 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #ifndef MGF_PP_TILE_ROWS
 #define MGF_PP_TILE_ROWS 32
 #endif
+/* tile rows of the merged relaxation's p' launch on small levels (16: one
+ * 64 x 64 tile per level, assembled once, was slower -- such a launch is
+ * latency-bound and fewer, longer workgroups lengthen it) */
+#ifndef MGF_SMALL_TILE_ROWS
+#define MGF_SMALL_TILE_ROWS MGF_SWEEP_TILE_ROWS
+#endif
 /* threads of the merged relaxation's p' launch (512 or 1024) */
 #ifndef MGF_PP_THREADS
 #define MGF_PP_THREADS 512
@@ -736,9 +742,9 @@ static void launch_pp_merged_s(const DevGroup &G, hipStream_t st)
     int nx, ny;
     sweep_extents(G, 2, nx, ny);
     if (small_group(G)) {
-        const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SWEEP_TILE_ROWS);
-        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM_CORR, 2>), g, dim3(BX, 16), 0, st, G.d, 2,
-                           0, 3);
+        const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SMALL_TILE_ROWS);
+        hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM_CORR, 2, MGF_SMALL_TILE_ROWS>), g,
+                           dim3(BX, 16), 0, st, G.d, 2, 0, 3);
     } else {
         const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_PP_TILE_ROWS);
         hipLaunchKernelGGL((k_rb32x<S, MGF_PP_THREADS, MODE_ASM_CORR, 2, MGF_PP_TILE_ROWS>), g,
