@@ -15,7 +15,8 @@ from mgfvm.inputs import Problem, channel_128_inline, parse
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "mgfvm.h")
 
-ERR = {"ARG": -1, "IO": -2, "PARSE": -3, "CONFIG": -4, "GEOMETRY": -5, "LIMIT": -6}
+ERR = {"ARG": -1, "IO": -2, "PARSE": -3, "CONFIG": -4, "GEOMETRY": -5, "LIMIT": -6,
+       "UNSUPPORTED": -9}
 
 
 def declared():
@@ -37,13 +38,15 @@ def test_abi_version():
     assert mgfvm.lib().mgf_abi_version() == 1
 
 
-def _create(prob: Problem, tmp_path, strict=False):
+def _create(prob: Problem, tmp_path, strict=False, **optkw):
     pin, ppat = prob.write(str(tmp_path))
     p = _lib.read_problem(pin, ppat)
     try:
         opt = _lib.Options()
         mgfvm.lib().mgf_options_default(ctypes.byref(opt))
         opt.strict_limits = 1 if strict else 0
+        for k, v in optkw.items():
+            setattr(opt, k, v)
         h = ctypes.c_void_p()
         rc = mgfvm.lib().mgf_create(p, ctypes.byref(opt), ctypes.byref(h))
         msg = mgfvm.lib().mgf_error_string().decode()
@@ -147,3 +150,31 @@ def test_sensitivity_library_exports():
     assert extra == sorted(tangent.EXPORTS)
     missing = [n for n in declared() + extra if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_options_defaults():
+    """The defaults a caller gets: FP64, graphs, red-black, compaction on,
+    automatic lanes and launch shapes."""
+    opt = _lib.Options()
+    mgfvm.lib().mgf_options_default(ctypes.byref(opt))
+    assert (opt.precision, opt.use_graph, opt.device, opt.strict_limits) == (0, 1, -1, 0)
+    assert (opt.sweeps_per_launch, opt.sweep_threads, opt.split_assembly) == (0, 0, 0)
+    assert (opt.gs_order, opt.compact, opt.lanes) == (0, 1, 0)
+
+
+@pytest.mark.parametrize("optkw,code", [
+    (dict(gs_order=2), "ARG"),                           # neither red-black nor lexicographic
+    (dict(gs_order=1, precision=1), "UNSUPPORTED"),      # the variant is FP64 only
+])
+def test_create_rejects_options(tmp_path, optkw, code):
+    """Option errors are reported before any device work (no GPU needed)."""
+    prob = channel_128_inline().replace(isolve=2)
+    rc, msg = _create(prob, tmp_path, **optkw)
+    assert rc == ERR[code], msg
+    assert msg
+
+
+def test_lexicographic_variant_row_limit(tmp_path):
+    prob = Problem(nfx=64, nfy=2048, ngrid=2, ig_obs=2, isolve=2, obstacles=[])
+    rc, msg = _create(prob, tmp_path, gs_order=1)
+    assert rc == ERR["UNSUPPORTED"] and "1024" in msg
