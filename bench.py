@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # 512^2 members): rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # `bench.py --roofline-only --reps 10`, each scaled by what the counter reads
 # for a 1 GiB device copy (profiles/traffic_relax_uvs_512x84.md)
-TRAFFIC = 2240.3e6
+TRAFFIC = 2240.2e6
 WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
             "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
             "each obstacle moved by a seeded quarter/half/full cell), Re in "
@@ -266,6 +266,21 @@ def main():
         # ---- roofline of the dominant kernel ------------------------------
         roof = kernel_roofline(probs, pairs, args.reps)
 
+        # ---- the same batch in FP64 (bit-exact with the reference) --------
+        batched_fp64 = None
+        if not args.no_plain:
+            bf = mgfvm.Batch(pairs, precision=mgfvm.PREC_FP64, device=0)
+            t0 = time.perf_counter()
+            nf, cf = bf.solve()
+            _sync(bf)
+            tf = time.perf_counter() - t0
+            bf.close()
+            batched_fp64 = {"solves_per_s": round(sum(cf) / tf, 3), "seconds": round(tf, 2),
+                            "converged": int(sum(cf)),
+                            "cycle_counts_equal_mixed": bool(list(nf) == list(ncyc))}
+            print(f"[bench] batched fp64: {sum(cf)} converged in {tf:.2f}s", file=sys.stderr,
+                  flush=True)
+
         # ---- plain FP64 GPU path: one problem at a time -----------------
         plain_value, plain_detail = None, None
         if not args.no_plain:
@@ -366,6 +381,7 @@ def main():
                        "plain_fp64_gpu_estimate": plain_detail,
                        "batched_mixed_vs_plain":
                            round(value / plain_value, 2) if plain_value else None,
+                       "batched_fp64": batched_fp64,
                        "single_1024": single,
                        "gradient_256": gradient},
             "roofline": roof, "cpu_baseline": cpu,
