@@ -156,9 +156,9 @@ void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
  * Gauss-Seidel (the labelled variant; rows <= LEX_MAX_ROWS) */
 #define LEX_MAX_ROWS 1024
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
-/* pending: mixed precision, u and v corrections still in du32 / dv32 (1),
- * and the sc correction in ds32 as well, with the updated p of the copy
- * origins already in tmp (2) */
+/* pending: mixed precision, u and v corrections still in du32 / dv32 (1);
+ * the merged relaxation's u + du, v + dv in uc, vc, the sc correction in
+ * ds32 and the updated p of the copy origins in tmp (2) */
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);

@@ -16,6 +16,11 @@
  * mass defect -div as right-hand side; pp is then widened to FP64.
  * The level-0 sum |div|, the update of u, v, p and every grid transfer stay
  * FP64.
+ *
+ * The launch sequence of a relaxation: k_rb32x_uvs (u, v, sc together, when
+ * the configuration allows; see mgf_launch_relax_merged_mixed) and one p'
+ * launch, or field by field through mgf_launch_relax_field_mixed (split
+ * assembly, Jacobi, outflow boundaries).
  */
 #include <hip/hip_runtime.h>
 
@@ -267,7 +272,8 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     const gflt *src = launch == 0 ? nullptr : ((launch & 1) ? L.db32 : L.da32);
     gflt *dst = (launch & 1) ? L.da32 : L.db32;
     /* the last launch of u / v leaves the correction pending in du32 / dv32
-     * (and of sc in ds32 when the update applies it) */
+     * (the merged launch: u + du, v + dv in uc, vc, written below, and sc's
+     * correction in ds32 for the update to apply) */
     if (last && field == 0) dst = L.du32;
     if (last && field == 1) dst = L.dv32;
     if (INL && field == 3) dst = L.ds32;
