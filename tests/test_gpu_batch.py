@@ -228,3 +228,41 @@ def test_lanes_result_neutral(tmp_path, precision, lanes, graph):
                                       bt.get(k, 0, name).view(np.uint64)), (rep, k, name)
     ref.close()
     bt.close()
+
+
+@pytest.mark.parametrize("names", [("ibc2", "ibc3", "ibc4", "inlets", "rand64", "ibeg")])
+def test_merged_relaxation_boundary_variants(tmp_path, names):
+    """Boundary variants in one mixed batch, red-black: p' Dirichlet rows
+    (ibc 2), periodic copies (ibc 3: merged relaxation, its update without
+    the fused restriction), outflow (ibc 4: field by field), discrete inlets
+    and outlets, a random mask on another grid size, an obstacle offset.  Each member bit for
+    bit as the field-by-field split-assembly batch, same cycle counts and
+    flags as the FP64 oracle, fields within the mixed tolerance."""
+    import cases
+    import mgfvm
+    probs, pairs = [], []
+    for k, name in enumerate(names):
+        p = cases.CASES[name].replace(isolve=2, nswpm=2, nswpp=4, nvcycle=30)
+        probs.append(p)
+        pairs.append(p.write(str(tmp_path / f"v{k}")))
+    _say(f"[gpu] merged relaxation boundary variants {names}")
+    split = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, split_assembly=True)
+    ns, cs = split.solve()
+    bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED)
+    nb, cb = bt.solve()
+    assert (list(nb), list(cb)) == (list(ns), list(cs))
+    for k, (p, (pin, ppat)) in enumerate(zip(probs, pairs)):
+        assert np.array_equal(split.history(k), bt.history(k)), k
+        for name in ("u", "v", "p", "pp", "sc", "apu", "apv"):
+            assert np.array_equal(split.get(k, 0, name).view(np.uint64),
+                                  bt.get(k, 0, name).view(np.uint64)), (k, name)
+        o = orc.Oracle(pin, ppat)
+        ho, no, co = o.solve(p.nvcycle, p.convergence)
+        assert (nb[k], cb[k]) == (no, co), k
+        np.testing.assert_allclose(bt.history(k), ho, rtol=1e-5, atol=1e-12)
+        for name in ("u", "v", "p", "sc"):
+            g, r = bt.get(k, 0, name), o.get(0, name)
+            assert np.abs(g - r).max() / max(np.abs(r).max(), 1e-300) < MIXED_FIELD_RTOL, (k, name)
+        o.close()
+    split.close()
+    bt.close()
