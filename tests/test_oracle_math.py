@@ -212,3 +212,107 @@ def test_lexicographic_gs_sweep_order(tmp_path):
     r_last = ap[8, 8] * x[8, 8] - (co["aw"][8, 8] * x[8, 7] + co["as"][8, 8] * x[7, 8] + su[8, 8])
     assert abs(r_last) < 1e-12
     o.close()
+
+
+def test_red_black_one_sweep_colour_order(tmp_path):
+    """One red-black sweep from zero: the colour swept first ((i + j) even)
+    sees only its source (x = su / ap), and the second colour's equations
+    hold exactly afterwards (all their neighbours are of the first colour,
+    which does not change again).  Swapping the colours fails both."""
+    prob = Problem(nfx=8, nfy=8, ngrid=1, ig_obs=1, isolve=2, nswpp=1, obstacles=[])
+    o = _oracle(tmp_path, prob)
+    co, ap, su = _random_system(o, 3)
+    o.set(0, "pp", np.zeros((10, 10)))
+    o.call("rbgs_pp", 0)
+    x = o.get(0, "pp")
+    for j in range(1, 9):
+        for i in range(1, 9):
+            if (i + j) % 2 == 0:
+                assert x[j, i] == su[j, i] / ap[j, i]
+            else:
+                s = (co["aw"][j, i] * x[j, i - 1] + co["ae"][j, i] * x[j, i + 1] +
+                     co["as"][j, i] * x[j - 1, i] + co["an"][j, i] * x[j + 1, i])
+                r = ap[j, i] * x[j, i] - (s + su[j, i])
+                assert abs(r) <= 1e-12 * (1 + abs(su[j, i])), (j, i, r)
+
+
+def test_masserror_stream_function_and_unit_face(tmp_path):
+    """The continuity defect (V6:1096-1125) vanishes exactly for face
+    velocities derived from a corner stream function (dx = dy = 1/16, integer
+    stream function: every product exact), and a single perturbed u face
+    moves su = -div by -/+ areax * delta in the two cells it separates."""
+    prob = Problem(nfx=16, nfy=16, ngrid=1, ig_obs=1, obstacles=[])
+    o = _oracle(tmp_path, prob)
+    ny, nx = o.get(0, "u").shape
+    ax, ay = o.get(0, "areax")[1, 1], o.get(0, "areay")[1, 1]
+    assert ax == 1.0 / 16 and ay == 1.0 / 16
+    rng = np.random.default_rng(4)
+    psi = rng.integers(-8, 9, size=(ny, nx)).astype(float)
+    u = np.zeros((ny, nx))
+    v = np.zeros((ny, nx))
+    u[1:, :] = (psi[1:, :] - psi[:-1, :]) / ax       # u[j,i] = (psi[j,i] - psi[j-1,i]) / ax
+    v[:, 1:] = -(psi[:, 1:] - psi[:, :-1]) / ay      # v[j,i] = -(psi[j,i] - psi[j,i-1]) / ay
+    o.set(0, "u", u)
+    o.set(0, "v", v)
+    o.set(0, "pp", np.full((ny, nx), 7.0))
+    assert o.call("masserror", 0) == 0.0
+    assert not np.any(o.get(0, "su")[1:-1, 1:-1])
+    assert not np.any(o.get(0, "pp")[1:-1, 1:-1])
+    delta = 0.375
+    u[5, 6] += delta
+    o.set(0, "u", u)
+    err = o.call("masserror", 0)
+    assert err == 2 * ax * delta
+    su = o.get(0, "su")
+    assert su[5, 6] == -ax * delta and su[5, 7] == ax * delta
+    su[5, 6] = su[5, 7] = 0.0
+    assert not np.any(su[1:-1, 1:-1])
+
+
+def _cell_centres(n, h):
+    return (np.arange(n) - 0.5) * h
+
+
+@pytest.mark.parametrize("field", ["p", "sc"])
+def test_cell_prolongation_reproduces_linear_corrections(tmp_path, field):
+    """Bilinear 9/3/3/1 prolongation (V6:1894-2028): with the fine field zero
+    (so the restricted fine value is zero), a coarse correction linear in x
+    and y reaches every fine cell whose four coarse neighbours are interior
+    as the same linear function at the fine cell centre.  A wrong neighbour
+    or a 9/16 weight on the wrong cell fails it."""
+    prob = Problem(nfx=16, nfy=16, ngrid=2, ig_obs=2, obstacles=[])
+    o = _oracle(tmp_path, prob)
+    nyf, nxf = o.get(0, field).shape
+    nyc, nxc = o.get(1, field).shape
+    hf, hc = 1.0 / 16, 1.0 / 8
+    lin = lambda x, y: 1.0 + 0.75 * x - 2.5 * y
+    yc, xc = np.meshgrid(_cell_centres(nyc, hc), _cell_centres(nxc, hc), indexing="ij")
+    o.set(0, field, np.zeros((nyf, nxf)))
+    o.set(1, field, lin(xc, yc))
+    o.call("prol" + field, 0)
+    got = o.get(0, field)
+    yf, xf = np.meshgrid(_cell_centres(nyf, hf), _cell_centres(nxf, hf), indexing="ij")
+    want = lin(xf, yf)
+    # fine cells 2..nf-3 have all four coarse neighbours inside 1..nc-2
+    np.testing.assert_allclose(got[2:-2, 2:-2], want[2:-2, 2:-2], rtol=0, atol=1e-14)
+
+
+def test_face_prolongation_reproduces_linear_corrections(tmp_path):
+    """Face prolongation of u (V6:1745-1820): 3/4-1/4 across rows at the
+    coincident faces, halves onto the faces between them; a linear coarse
+    correction reaches interior fine faces as the same linear function."""
+    prob = Problem(nfx=16, nfy=16, ngrid=2, ig_obs=2, obstacles=[])
+    o = _oracle(tmp_path, prob)
+    nyf, nxf = o.get(0, "u").shape
+    nyc, nxc = o.get(1, "u").shape
+    hf, hc = 1.0 / 16, 1.0 / 8
+    lin = lambda x, y: -0.5 + 1.25 * x + 3.0 * y
+    yc, xc = np.meshgrid(_cell_centres(nyc, hc), np.arange(nxc) * hc, indexing="ij")
+    o.set(0, "u", np.zeros((nyf, nxf)))
+    o.set(1, "u", lin(xc, yc))
+    o.call("prolu", 0)
+    got = o.get(0, "u")
+    yf, xf = np.meshgrid(_cell_centres(nyf, hf), np.arange(nxf) * hf, indexing="ij")
+    want = lin(xf, yf)
+    # fine rows 2..nyf-3, faces 3..nxf-5: both coarse rows and both coarse faces interior
+    np.testing.assert_allclose(got[2:-2, 3:-4], want[2:-2, 3:-4], rtol=0, atol=1e-14)
