@@ -167,7 +167,10 @@ void mgf_launch_restrict(const DevPair &R, int restv, hipStream_t st);
 /* the update of a down-leg group fused with restv onto its coarser levels
  * (R: the transfer pair of the same levels, members in the same order) */
 void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, hipStream_t st);
-void mgf_launch_prolong(const DevPair &R, hipStream_t st);
+/* stored: c = x_c - R x_f with R x_f as the restriction stored it (a
+ * V-cycle, whose fine level has not changed since); 0: R x_f recomputed from
+ * the fine level (the step-level call) */
+void mgf_launch_prolong(const DevPair &R, int stored, hipStream_t st);
 /* the stopping test for the n instances listed in members (device array) */
 void mgf_launch_end_cycle(DevCtl *ctls, const int *members, int n, hipStream_t st);
 /* mixed precision: FP64 assembly + residual, FP32 correction sweeps, FP64 apply */

@@ -433,10 +433,20 @@ __global__ void __launch_bounds__(BX * BY) k_update_restv(const DevLevel *Fs, co
     /* this thread's cell is (2ic, 2jc): ij; its block's ij-1, ij-nxf, ij-nxf-1 */
     const int ic = i / 2, jc = j / 2;
     const int ijc = jc * C.npx + ic;
-    C.u[ijc] = (c.u * F.ax + X[0][ty - 1][tx] * F.ax) / (C.ax + 1.0e-30);
-    C.v[ijc] = (c.v * F.ay + X[1][ty][tx - 1] * F.ay) / (C.ay + 1.0e-30);
-    C.p[ijc] = 0.25 * (((c.p + X[2][ty][tx - 1]) + X[2][ty - 1][tx]) + X[2][ty - 1][tx - 1]);
-    C.sc[ijc] = 0.25 * (((c.sc + X[3][ty][tx - 1]) + X[3][ty - 1][tx]) + X[3][ty - 1][tx - 1]);
+    const real_t ru = (c.u * F.ax + X[0][ty - 1][tx] * F.ax) / (C.ax + 1.0e-30);
+    const real_t rv = (c.v * F.ay + X[1][ty][tx - 1] * F.ay) / (C.ay + 1.0e-30);
+    const real_t rp = 0.25 * (((c.p + X[2][ty][tx - 1]) + X[2][ty - 1][tx]) + X[2][ty - 1][tx - 1]);
+    const real_t rs =
+        0.25 * (((c.sc + X[3][ty][tx - 1]) + X[3][ty - 1][tx]) + X[3][ty - 1][tx - 1]);
+    C.u[ijc] = ru;
+    C.v[ijc] = rv;
+    C.p[ijc] = rp;
+    C.sc[ijc] = rs;
+    /* R x_f, kept for the V-cycle's prolongation (k_prol_s) */
+    C.cu[ijc] = ru;
+    C.cv[ijc] = rv;
+    C.cp[ijc] = rp;
+    C.csc[ijc] = rs;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -489,7 +499,12 @@ __global__ void k_outflow(const DevLevel *Ls)
 /* restriction                                                              */
 /* ---------------------------------------------------------------------- */
 
-/* restv, V6:1354-1389: coarse [1,nyc-2]x[1,nxc-2] */
+/* restv, V6:1354-1389: coarse [1,nyc-2]x[1,nxc-2].  The restricted values
+ * R x_f are also kept in cu, cv, cp, csc: the fine level does not change
+ * again before the V-cycle prolongs onto it, and the prolongation's R x_f
+ * (V6:1762-1995) is the same expression on the same values, so the stored
+ * copy is that R x_f bit for bit (k_prol_s reads it instead of
+ * recomputing it). */
 __global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
@@ -499,10 +514,18 @@ __global__ void __launch_bounds__(BX * BY) k_restv(const DevLevel *Fs, const Dev
     const int nxf = F.npx;
     const int ijc = jc * C.npx + ic;
     const int ij = (2 * jc) * nxf + 2 * ic, ijw = ij - 1, ijs = ij - nxf;
-    C.u[ijc] = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0e-30);
-    C.v[ijc] = (F.v[ij] * F.ay + F.v[ijw] * F.ay) / (C.ay + 1.0e-30);
-    C.p[ijc] = 0.25 * (((F.p[ij] + F.p[ijw]) + F.p[ijs]) + F.p[ijs - 1]);
-    C.sc[ijc] = 0.25 * (((F.sc[ij] + F.sc[ijw]) + F.sc[ijs]) + F.sc[ijs - 1]);
+    const real_t ru = (F.u[ij] * F.ax + F.u[ijs] * F.ax) / (C.ax + 1.0e-30);
+    const real_t rv = (F.v[ij] * F.ay + F.v[ijw] * F.ay) / (C.ay + 1.0e-30);
+    const real_t rp = 0.25 * (((F.p[ij] + F.p[ijw]) + F.p[ijs]) + F.p[ijs - 1]);
+    const real_t rs = 0.25 * (((F.sc[ij] + F.sc[ijw]) + F.sc[ijs]) + F.sc[ijs - 1]);
+    C.u[ijc] = ru;
+    C.v[ijc] = rv;
+    C.p[ijc] = rp;
+    C.sc[ijc] = rs;
+    C.cu[ijc] = ru;
+    C.cv[ijc] = rv;
+    C.cp[ijc] = rp;
+    C.csc[ijc] = rs;
 }
 
 /* where the residuals take u and v from, relative to their point (di, dj):
@@ -618,6 +641,17 @@ __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLev
     if (ic0 > nxc - 2 || jc0 > nyc - 2) return;          /* padding tiles */
     const int fi0 = 2 * ic0 - 1, fj0 = 2 * jc0 - 1;
     const int tid = threadIdx.y * BX + threadIdx.x;
+    /* the coarse residuals first: their loads overlap the fine phase's
+     * instead of waiting behind its barrier */
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int ic = ic0 + tx, jc = jc0 + ty;
+    const bool own = ty < BY && ic <= nxc - 2 && jc <= nyc - 2;
+    real_t rcu = 0.0, rcv = 0.0, rcs = 0.0;
+    if (own) {
+        if (ic <= nxc - 3) rcu = rs_u(C, ic, jc, st_g(C, ic, jc), C.bodyx, false);
+        if (jc <= nyc - 3) rcv = rs_v(C, ic, jc, st_g(C, ic, jc), C.bodyy, false);
+        rcs = rs_sc(C, ic, jc, st_g(C, ic, jc), false);
+    }
     if (STAGE) {
         const int nxf = F.npx, nyf = F.npy;
         for (int e = tid; e < TAU_SH * TAU_SW; e += TAU_NT) {
@@ -662,30 +696,25 @@ __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLev
         Q[r][c] = (c <= cq && r <= rq) ? rs_sc(F, fi0 + c, fj0 + r, fst(c, r), true, m) : 0.0;
     }
     __syncthreads();
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int ic = ic0 + tx, jc = jc0 + ty;
-    if (ty >= BY || ic > nxc - 2 || jc > nyc - 2) return;
+    if (!own) return;
     const int ijc = jc * nxc + ic;
     const int c1 = 2 * tx + 1, r1 = 2 * ty + 1;     /* fine (2ic, 2jc) in the tables */
     if (ic <= nxc - 3) {
         real_t a = U[r1][c1], b = U[r1 - 1][c1];
         real_t w0 = U[r1][c1 - 1], w1 = U[r1 - 1][c1 - 1];
         real_t e0 = U[r1][c1 + 1], e1 = U[r1 - 1][c1 + 1];
-        real_t rc = rs_u(C, ic, jc, st_g(C, ic, jc), C.bodyx, false);
-        C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rc;
+        C.resu[ijc] = ((a + b) + 0.5 * (((w0 + w1) + e0) + e1)) - rcu;
     }
     if (jc <= nyc - 3) {
         real_t a = V[r1][c1], b = V[r1][c1 - 1];
         real_t s0 = V[r1 - 1][c1], s1 = V[r1 - 1][c1 - 1];
         real_t n0 = V[r1 + 1][c1], n1 = V[r1 + 1][c1 - 1];
-        real_t rc = rs_v(C, ic, jc, st_g(C, ic, jc), C.bodyy, false);
-        C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rc;
+        C.resv[ijc] = ((a + b) + 0.5 * (((s0 + s1) + n0) + n1)) - rcv;
     }
     {
         real_t a = Q[r1][c1], b = Q[r1 - 1][c1];
         real_t c2 = Q[r1][c1 - 1], d = Q[r1 - 1][c1 - 1];
-        real_t rc = rs_sc(C, ic, jc, st_g(C, ic, jc), false);
-        C.ressc[ijc] = (((a + b) + c2) + d) - rc;
+        C.ressc[ijc] = (((a + b) + c2) + d) - rcs;
     }
 }
 
@@ -695,8 +724,9 @@ __global__ void __launch_bounds__(TAU_NT) k_tau(const DevLevel *Fs, const DevLev
 
 /* coarse corrections c = x_c - R x_f for u, v, p, sc (prolu 1762-1779,
  * prolv 1838-1855, prolp 1911-1928, prolsc 1979-1995).  All four read only
- * pre-prolongation fine data, so one pass serves them all; the entries the
- * reference leaves at zero are never written here (zero since creation). */
+ * pre-prolongation fine data, so one pass serves them all.  The entries the
+ * reference leaves at zero (ghosts; cu of the last face column, cv of the
+ * last face row) are never read by the prolongation. */
 __global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const DevLevel *Cs)
 {
     PAIR(Fs, Cs);
@@ -734,28 +764,52 @@ __global__ void __launch_bounds__(BX * BY) k_prol_c(const DevLevel *Fs, const De
     }
 }
 
+/* where the application takes the coarse corrections from: the level's
+ * cu, cv, cp, csc as k_prol_c left them, with the fine face flags read from
+ * the fine level ... */
+struct CorrG {
+    const DevLevelG &F, &C;
+    __device__ real_t u(int ic, int jc) const { return C.cu[jc * C.npx + ic]; }
+    __device__ real_t v(int ic, int jc) const { return C.cv[jc * C.npx + ic]; }
+    __device__ real_t p(int ic, int jc) const { return C.cp[jc * C.npx + ic]; }
+    __device__ real_t s(int ic, int jc) const { return C.csc[jc * C.npx + ic]; }
+    /* flags of the fine point (2ic, 2jc) */
+    __device__ uint8_t ff(int ic, int jc) const { return F.flags[(2 * jc) * F.npx + 2 * ic]; }
+};
+
+/* ... or a workgroup's LDS tile of them, coarse points (ic0 - 1 .. ic0 + BX,
+ * jc0 - 1 .. jc0 + BY), formed by k_prol_s */
+#define PROL_W (BX + 2)
+#define PROL_H (BY + 2)
+struct CorrL {
+    const real_t (*cu)[PROL_W], (*cv)[PROL_W], (*cp)[PROL_W], (*cs)[PROL_W];
+    const uint8_t (*fl)[PROL_W];
+    int ic0, jc0;
+    __device__ real_t u(int ic, int jc) const { return cu[jc - jc0 + 1][ic - ic0 + 1]; }
+    __device__ real_t v(int ic, int jc) const { return cv[jc - jc0 + 1][ic - ic0 + 1]; }
+    __device__ real_t p(int ic, int jc) const { return cp[jc - jc0 + 1][ic - ic0 + 1]; }
+    __device__ real_t s(int ic, int jc) const { return cs[jc - jc0 + 1][ic - ic0 + 1]; }
+    __device__ uint8_t ff(int ic, int jc) const { return fl[jc - jc0 + 1][ic - ic0 + 1]; }
+};
+
 /* face correction of prolu/prolv for coarse face k along the face direction,
  * mixing in the neighbour across it (3/4, 1/4), zero where the fine face at
  * (2ic, 2jc) is penalised (V6:1794-1798, 1870-1874) */
-__device__ __forceinline__ real_t corr_u(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
-                                         bool upper)
+template <class A>
+__device__ __forceinline__ real_t corr_u(const A &a, int ic, int jc, bool upper)
 {
-    const int nxc = C.npx;
-    const int ijc = jc * nxc + ic;
     const double f1 = 3. / 4., f2 = 1. / 4.;
-    real_t c = upper ? f1 * C.cu[ijc] + f2 * C.cu[ijc + nxc] : f1 * C.cu[ijc] + f2 * C.cu[ijc - nxc];
-    if (F.flags[(2 * jc) * F.npx + 2 * ic] & MGF_F_SPU) c = 0.0;
+    real_t c = upper ? f1 * a.u(ic, jc) + f2 * a.u(ic, jc + 1) : f1 * a.u(ic, jc) + f2 * a.u(ic, jc - 1);
+    if (a.ff(ic, jc) & MGF_F_SPU) c = 0.0;
     return c;
 }
 
-__device__ __forceinline__ real_t corr_v(const DevLevelG &F, const DevLevelG &C, int ic, int jc,
-                                         bool right)
+template <class A>
+__device__ __forceinline__ real_t corr_v(const A &a, int ic, int jc, bool right)
 {
-    const int nxc = C.npx;
-    const int ijc = jc * nxc + ic;
     const double f1 = 3. / 4., f2 = 1. / 4.;
-    real_t c = right ? f1 * C.cv[ijc] + f2 * C.cv[ijc + 1] : f1 * C.cv[ijc] + f2 * C.cv[ijc - 1];
-    if (F.flags[(2 * jc) * F.npx + 2 * ic] & MGF_F_SPV) c = 0.0;
+    real_t c = right ? f1 * a.v(ic, jc) + f2 * a.v(ic + 1, jc) : f1 * a.v(ic, jc) + f2 * a.v(ic - 1, jc);
+    if (a.ff(ic, jc) & MGF_F_SPV) c = 0.0;
     return c;
 }
 
@@ -769,17 +823,34 @@ __device__ __forceinline__ real_t corr_cell_n(const real_t n[3][3], int di, int 
 }
 
 /* Every fine point gathers its corrections; a thread takes the four fine
- * cells (and their faces) of one coarse cell, so the coarse values are
- * loaded once for four points.  Odd faces, which the reference updates
- * twice (atomics in its GPU build), take the two halves in its CPU loop
- * order (smaller coarse index first). */
-__global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, const DevLevel *Cs)
+ * cells (and their faces) of coarse cell (ic, jc), so the coarse values are
+ * read once for four points.  Odd faces, which the reference updates twice
+ * (atomics in its GPU build), take the two halves in its CPU loop order
+ * (smaller coarse index first). */
+struct Fine16 {
+    real_t u[4], v[4], p[4], s[4];
+};
+
+/* the fine values the thread of coarse cell (ic, jc) updates (each fine
+ * point belongs to exactly one coarse cell, so they can be read early) */
+__device__ __forceinline__ Fine16 prol_load(const DevLevelG &F, int ic, int jc)
 {
-    PAIR(Fs, Cs);
-    const int ic = 1 + (int)(T.x * BX + threadIdx.x);
-    const int jc = 1 + (int)(T.y * BY + threadIdx.y);
+    const int nxf = F.npx;
+    const int i1 = 2 * ic - 1, i2 = 2 * ic;
+    const int r1 = (2 * jc - 1) * nxf, r2 = (2 * jc) * nxf;
+    Fine16 x;
+    x.u[0] = F.u[r2 + i1]; x.u[1] = F.u[r1 + i1]; x.u[2] = F.u[r2 + i2]; x.u[3] = F.u[r1 + i2];
+    x.v[0] = F.v[r1 + i2]; x.v[1] = F.v[r1 + i1]; x.v[2] = F.v[r2 + i2]; x.v[3] = F.v[r2 + i1];
+    x.p[0] = F.p[r1 + i1]; x.p[1] = F.p[r1 + i2]; x.p[2] = F.p[r2 + i1]; x.p[3] = F.p[r2 + i2];
+    x.s[0] = F.sc[r1 + i1]; x.s[1] = F.sc[r1 + i2]; x.s[2] = F.sc[r2 + i1]; x.s[3] = F.sc[r2 + i2];
+    return x;
+}
+
+template <class A>
+__device__ __forceinline__ void prol_apply_at(const DevLevelG &F, const DevLevelG &C, int ic,
+                                              int jc, const A &a, const Fine16 &x)
+{
     const int nxf = F.npx, nxc = C.npx, nyc = C.npy;
-    if (ic > nxc - 2 || jc > nyc - 2) return;
     const int i1 = 2 * ic - 1, i2 = 2 * ic;
     const int r1 = (2 * jc - 1) * nxf, r2 = (2 * jc) * nxf;
     /* u faces i1 (odd: halves from coarse faces ic-1 and ic) and i2 = 2ic
@@ -788,19 +859,19 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, cons
     {
         const bool hr = ic <= nxc - 3, hl = ic >= 2;
         real_t crU = 0.0, crL = 0.0, clU = 0.0, clL = 0.0;
-        if (hr) { crU = corr_u(F, C, ic, jc, true); crL = corr_u(F, C, ic, jc, false); }
-        if (hl) { clU = corr_u(F, C, ic - 1, jc, true); clL = corr_u(F, C, ic - 1, jc, false); }
-        real_t u = F.u[r2 + i1];
+        if (hr) { crU = corr_u(a, ic, jc, true); crL = corr_u(a, ic, jc, false); }
+        if (hl) { clU = corr_u(a, ic - 1, jc, true); clL = corr_u(a, ic - 1, jc, false); }
+        real_t u = x.u[0];
         if (hl) u = u + 0.5 * clU;
         if (hr) u = u + 0.5 * crU;
         F.u[r2 + i1] = u;
-        u = F.u[r1 + i1];
+        u = x.u[1];
         if (hl) u = u + 0.5 * clL;
         if (hr) u = u + 0.5 * crL;
         F.u[r1 + i1] = u;
         if (hr) {
-            F.u[r2 + i2] = F.u[r2 + i2] + crU;
-            F.u[r1 + i2] = F.u[r1 + i2] + crL;
+            F.u[r2 + i2] = x.u[2] + crU;
+            F.u[r1 + i2] = x.u[3] + crL;
         }
     }
     /* v faces: row 2jc-1 (odd: halves from coarse faces jc-1 and jc) and row
@@ -808,41 +879,105 @@ __global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, cons
     {
         const bool hu = jc <= nyc - 3, hd = jc >= 2;
         real_t cuR = 0.0, cuL = 0.0, cdR = 0.0, cdL = 0.0;
-        if (hu) { cuR = corr_v(F, C, ic, jc, true); cuL = corr_v(F, C, ic, jc, false); }
-        if (hd) { cdR = corr_v(F, C, ic, jc - 1, true); cdL = corr_v(F, C, ic, jc - 1, false); }
-        real_t v = F.v[r1 + i2];
+        if (hu) { cuR = corr_v(a, ic, jc, true); cuL = corr_v(a, ic, jc, false); }
+        if (hd) { cdR = corr_v(a, ic, jc - 1, true); cdL = corr_v(a, ic, jc - 1, false); }
+        real_t v = x.v[0];
         if (hd) v = v + 0.5 * cdR;
         if (hu) v = v + 0.5 * cuR;
         F.v[r1 + i2] = v;
-        v = F.v[r1 + i1];
+        v = x.v[1];
         if (hd) v = v + 0.5 * cdL;
         if (hu) v = v + 0.5 * cuL;
         F.v[r1 + i1] = v;
         if (hu) {
-            F.v[r2 + i2] = F.v[r2 + i2] + cuR;
-            F.v[r2 + i1] = F.v[r2 + i1] + cuL;
+            F.v[r2 + i2] = x.v[2] + cuR;
+            F.v[r2 + i1] = x.v[3] + cuL;
         }
     }
     /* cells */
     {
-        const int ijc = jc * nxc + ic;
         real_t np[3][3], ns[3][3];
 #pragma unroll
         for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
             for (int di = -1; di <= 1; ++di) {
-                np[dj + 1][di + 1] = C.cp[ijc + dj * nxc + di];
-                ns[dj + 1][di + 1] = C.csc[ijc + dj * nxc + di];
+                np[dj + 1][di + 1] = a.p(ic + di, jc + dj);
+                ns[dj + 1][di + 1] = a.s(ic + di, jc + dj);
             }
-        F.p[r1 + i1] = F.p[r1 + i1] + corr_cell_n(np, -1, -1);
-        F.p[r1 + i2] = F.p[r1 + i2] + corr_cell_n(np, 1, -1);
-        F.p[r2 + i1] = F.p[r2 + i1] + corr_cell_n(np, -1, 1);
-        F.p[r2 + i2] = F.p[r2 + i2] + corr_cell_n(np, 1, 1);
-        F.sc[r1 + i1] = F.sc[r1 + i1] + corr_cell_n(ns, -1, -1);
-        F.sc[r1 + i2] = F.sc[r1 + i2] + corr_cell_n(ns, 1, -1);
-        F.sc[r2 + i1] = F.sc[r2 + i1] + corr_cell_n(ns, -1, 1);
-        F.sc[r2 + i2] = F.sc[r2 + i2] + corr_cell_n(ns, 1, 1);
+        F.p[r1 + i1] = x.p[0] + corr_cell_n(np, -1, -1);
+        F.p[r1 + i2] = x.p[1] + corr_cell_n(np, 1, -1);
+        F.p[r2 + i1] = x.p[2] + corr_cell_n(np, -1, 1);
+        F.p[r2 + i2] = x.p[3] + corr_cell_n(np, 1, 1);
+        F.sc[r1 + i1] = x.s[0] + corr_cell_n(ns, -1, -1);
+        F.sc[r1 + i2] = x.s[1] + corr_cell_n(ns, 1, -1);
+        F.sc[r2 + i1] = x.s[2] + corr_cell_n(ns, -1, 1);
+        F.sc[r2 + i2] = x.s[3] + corr_cell_n(ns, 1, 1);
     }
+}
+
+__global__ void __launch_bounds__(BX * BY) k_prol_apply(const DevLevel *Fs, const DevLevel *Cs)
+{
+    PAIR(Fs, Cs);
+    const int ic = 1 + (int)(T.x * BX + threadIdx.x);
+    const int jc = 1 + (int)(T.y * BY + threadIdx.y);
+    if (ic > C.npx - 2 || jc > C.npy - 2) return;
+    prol_apply_at(F, C, ic, jc, CorrG{F, C}, prol_load(F, ic, jc));
+}
+
+/* The V-cycle's prolongation in one launch.  cu, cv, cp, csc hold R x_f as
+ * the restriction stored it (k_restv, k_update_restv), which is the R x_f
+ * the reference recomputes here, so c = x_c - R x_f is formed per
+ * workgroup, with k_prol_c's masks and zero where k_prol_c leaves zero, for
+ * the workgroup's coarse points and a one-point ring around them, in LDS;
+ * then applied as by k_prol_apply.  Not for a prolongation whose fine level
+ * changed after the restriction (the step-level call uses k_prol_c). */
+__global__ void __launch_bounds__(BX * BY) k_prol_s(const DevLevel *Fs, const DevLevel *Cs)
+{
+    PAIR(Fs, Cs);
+    __shared__ real_t Su[PROL_H][PROL_W], Sv[PROL_H][PROL_W];
+    __shared__ real_t Sp[PROL_H][PROL_W], Ss[PROL_H][PROL_W];
+    __shared__ uint8_t Sf[PROL_H][PROL_W];
+    const int nxc = C.npx, nyc = C.npy, nxf = F.npx;
+    const int ic0 = 1 + T.x * BX, jc0 = 1 + T.y * BY;
+    if (ic0 > nxc - 2 || jc0 > nyc - 2) return;          /* padding tiles */
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int ic = ic0 + (int)threadIdx.x, jc = jc0 + (int)threadIdx.y;
+    const bool own = ic <= nxc - 2 && jc <= nyc - 2;
+    /* the thread's fine values first, in flight across the barrier */
+    Fine16 x{};
+    if (own) x = prol_load(F, ic, jc);
+    for (int e = tid; e < PROL_H * PROL_W; e += BX * BY) {
+        const int r = e / PROL_W, q = e - r * PROL_W;
+        const int ic = ic0 - 1 + q, jc = jc0 - 1 + r;
+        real_t cu = 0.0, cv = 0.0, cp = 0.0, cs = 0.0;
+        uint8_t ffl = 0;
+        if (ic >= 1 && ic <= nxc - 2 && jc >= 1 && jc <= nyc - 2) {
+            const int ijc = jc * nxc + ic;
+            const uint8_t cfl = C.ext ? C.flags[ijc] : (uint8_t)0;
+            ffl = F.flags[(2 * jc) * nxf + 2 * ic];
+            if (ic <= nxc - 3) {
+                cu = C.u[ijc] - C.cu[ijc];
+                if (cfl & MGF_F_SPU) cu = 0.0;
+            }
+            if (jc <= nyc - 3) {
+                cv = C.v[ijc] - C.cv[ijc];
+                if (cfl & MGF_F_SPV) cv = 0.0;
+            }
+            cp = C.p[ijc] - C.cp[ijc];
+            if (ffl & MGF_F_SPU) cp = 0.0;
+            if (cfl & MGF_F_SPSC) cp = 0.0;
+            cs = C.sc[ijc] - C.csc[ijc];
+            if (cfl & MGF_F_SPSC) cs = 0.0;
+        }
+        Su[r][q] = cu;
+        Sv[r][q] = cv;
+        Sp[r][q] = cp;
+        Ss[r][q] = cs;
+        Sf[r][q] = ffl;
+    }
+    __syncthreads();
+    if (!own) return;
+    prol_apply_at(F, C, ic, jc, CorrL{Su, Sv, Sp, Ss, Sf, ic0, jc0}, x);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1050,12 +1185,15 @@ void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, h
         hipLaunchKernelGGL(k_update_restv<0>, g, dim3(BX, BY), 0, st, G.d, R.c);
 }
 
-void mgf_launch_prolong(const DevPair &R, hipStream_t st)
+void mgf_launch_prolong(const DevPair &R, int stored, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_prol_c, tile_grid(R.npxc - 2, R.npyc - 2, R.count), dim3(BX, BY), 0, st,
-                       R.f, R.c);
-    hipLaunchKernelGGL(k_prol_apply, tile_grid(R.npxc - 2, R.npyc - 2, R.count), dim3(BX, BY), 0,
-                       st, R.f, R.c);
+    const dim3 g = tile_grid(R.npxc - 2, R.npyc - 2, R.count);
+    if (stored) {
+        hipLaunchKernelGGL(k_prol_s, g, dim3(BX, BY), 0, st, R.f, R.c);
+        return;
+    }
+    hipLaunchKernelGGL(k_prol_c, g, dim3(BX, BY), 0, st, R.f, R.c);
+    hipLaunchKernelGGL(k_prol_apply, g, dim3(BX, BY), 0, st, R.f, R.c);
 }
 
 void mgf_launch_end_cycle(DevCtl *ctls, const int *members, int n, hipStream_t st)

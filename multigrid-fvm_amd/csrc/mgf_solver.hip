@@ -761,7 +761,7 @@ static void enqueue_vcycle(mgf_solver *s, const Program &P)
         }
         case OP_RELAX_UP: enqueue_relax(s, P.groups[o.g], 1, 0); break;
         case OP_RESTRICT: mgf_launch_restrict(P.pairs[o.g], !restv_done[o.g], s->est); break;
-        case OP_PROLONG: mgf_launch_prolong(P.pairs[o.g], s->est); break;
+        case OP_PROLONG: mgf_launch_prolong(P.pairs[o.g], 1, s->est); break;
         }
     }
 }
@@ -868,7 +868,7 @@ extern "C" int mgf_prolong(mgf_solver *s, int igc)
     if (rc) return rc;
     if (igc < 1 || igc >= s->inst[0].ng) return set_err(s, MGF_ERR_ARG, "bad coarse level");
     if ((rc = reset_ctl(s, 0, NULL, NULL))) return rc;
-    mgf_launch_prolong(s->full.pairs[s->level_pair[igc - 1]], s->st);
+    mgf_launch_prolong(s->full.pairs[s->level_pair[igc - 1]], 0, s->st);
     if ((rc = check_launch(s))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
     return MGF_OK;
