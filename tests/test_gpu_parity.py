@@ -85,6 +85,30 @@ def test_step_by_step_vcycle(tmp_path, name):
                 assert not bad, f"cycle {cyc} relax up L{ig-1}:\n" + "\n".join(bad)
 
 
+@pytest.mark.parametrize("name", ["small_rb", "coarse_free"])
+def test_prolong_after_fine_level_changed(tmp_path, name):
+    """Step calls in an order no V-cycle uses: the fine level relaxes again
+    between its restriction and the prolongation onto it.  mgf_prolong then
+    recomputes R x_f from the changed fine level, as the reference does (the
+    V-cycle's prolongation reuses the copy its restriction kept, valid only
+    because a V-cycle never changes the fine level in between); a V-cycle
+    afterwards must still match."""
+    _say(f"[gpu] prolong after a fine relax {name}")
+    gpu, ora = _pair(tmp_path, name)
+    for g in (gpu, ora):
+        g.relax(0, 0)
+        g.restrict(0)
+        g.relax(1, 0)
+        g.relax(0, 1)
+        g.prolong(1)
+    bad = _diff_report(gpu, ora, [0, 1])
+    assert not bad, "\n".join(bad)
+    eg, eo = gpu.vcycle(0), ora.vcycle(0)
+    assert eg == pytest.approx(eo, rel=HIST_RTOL, abs=0.0)
+    bad = _diff_report(gpu, ora, range(ora.ngrid))
+    assert not bad, "\n".join(bad)
+
+
 @pytest.mark.parametrize("name", ["small_jac", "small_rb", "inlets", "ibc3", "rand64"])
 def test_vcycles(tmp_path, name):
     _say(f"[gpu] vcycles {name}")
