@@ -65,6 +65,8 @@ struct DevPhys {
     int ext;              /* level below the obstacle level: coarsened obstacles */   \
     int nblk;             /* workgroups of the residual kernel (partials) */          \
     int isolve;           /* 1 Jacobi, 2 red-black */                                 \
+    int pset;             /* mixed: u, v, sc are swept (nswpm >= 1), so penalised     \
+                             points are set, not corrected (pending_value) */         \
     /* metrics, each computed on the host with the reference's expression */         \
     real_t ax, ay;        /* areax (= dy), areay (= dx) */                            \
     real_t du_x, du_y;    /* amu*(areax/dx), amu*(areay/dy): coefu, coefv, walls */   \

@@ -339,7 +339,8 @@ __global__ void k_pstash(const DevLevel *Ls)
 
 /* PENDING >= 1: mixed precision, where the corrections of the u and v
  * solves are still in du32 / dv32 over their swept ranges (the value of u
- * there is u + du, formed exactly as the solve's own update would); 2: the
+ * there is u + du, formed exactly as the solve's own update would, and 0 on
+ * penalised faces: pending_value); 2: the
  * sc solve ran in the same launch as u and v (which left u + du, v + dv in
  * uc, vc), and its correction in ds32 is
  * applied here (sc += d, as k_apply_mx would; the update does not read sc),
@@ -357,6 +358,7 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     const int nx = L.npx;
     const int ij = j * nx + i;
     const real_t pp = L.pp[ij];
+    const uint8_t fl = L.flags[ij];
     real_t u, v;
     if (PENDING == 2) {
         /* the merged relaxation formed u + du, v + dv in uc, vc */
@@ -367,16 +369,16 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
         v = L.v[ij];
         if (PENDING) {
             const float du = L.du32[ij], dv = L.dv32[ij];
-            u = i <= nx - 3 ? u + (double)du : u;
-            v = j <= L.npy - 3 ? v + (double)dv : v;
+            u = i <= nx - 3 ? pending_value(u, du, L.pset && (fl & MGF_F_SPU), 0.0) : u;
+            v = j <= L.npy - 3 ? pending_value(v, dv, L.pset && (fl & MGF_F_SPV), 0.0) : v;
         }
     }
     Cell4 c;
     c.u = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
     c.v = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
     c.sc = L.sc[ij];
-    if (PENDING == 2) c.sc = c.sc + (double)L.ds32[ij];
-    if (L.flags[ij] & MGF_F_PCOPY) {
+    if (PENDING == 2) c.sc = pending_value(c.sc, L.ds32[ij], L.pset && (fl & MGF_F_SPSC), P.sc_obs);
+    if (fl & MGF_F_PCOPY) {
         if (PENDING == 2) {
             /* the origin's updated p, stashed by the last p' launch; an
              * origin outside the updated range keeps its p */

@@ -105,8 +105,9 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int
 /* masserror + coefp in FP64 (V6:1018-1125) at one cell, rounded to FP32;
  * mag = |div|.  u and v carry their pending corrections over their swept
  * ranges (u faces 1..npx-3, v faces 1..npy-3): the values are u + du,
- * v + dv, formed as the solves' own update would form them -- here (CORR
- * false), or already formed by the merged relaxation in uc, vc (CORR). */
+ * v + dv (pending_value: 0 on penalised faces), formed as the solves' own
+ * update would form them -- here (CORR false), or already formed by the
+ * merged relaxation in uc, vc (CORR). */
 template <bool CORR = false>
 __device__ __forceinline__ Pt32 asm_pt_p(const DevLevelG &L, int i, int j, double &mag)
 {
@@ -122,10 +123,11 @@ __device__ __forceinline__ Pt32 asm_pt_p(const DevLevelG &L, int i, int j, doubl
     } else {
         const double u0 = L.u[ij], u1 = L.u[ij - 1], v0 = L.v[ij], v1 = L.v[ijs];
         const float du0 = L.du32[ij], du1 = L.du32[ij - 1], dv0 = L.dv32[ij], dv1 = L.dv32[ijs];
-        ue = i <= nx - 3 ? u0 + (double)du0 : u0;
-        uw = i - 1 >= 1 ? u1 + (double)du1 : u1;
-        vn = j <= L.npy - 3 ? v0 + (double)dv0 : v0;
-        vs = j - 1 >= 1 ? v1 + (double)dv1 : v1;
+        const uint8_t f0 = L.flags[ij], fw = L.flags[ij - 1], fs = L.flags[ijs];
+        ue = i <= nx - 3 ? pending_value(u0, du0, L.pset && (f0 & MGF_F_SPU), 0.0) : u0;
+        uw = i - 1 >= 1 ? pending_value(u1, du1, L.pset && (fw & MGF_F_SPU), 0.0) : u1;
+        vn = j <= L.npy - 3 ? pending_value(v0, dv0, L.pset && (f0 & MGF_F_SPV), 0.0) : v0;
+        vs = j - 1 >= 1 ? pending_value(v1, dv1, L.pset && (fs & MGF_F_SPV), 0.0) : v1;
     }
     double div = ((L.ax * ue - L.ax * uw) + L.ay * vn) - L.ay * vs;
     double su = -div;
@@ -360,9 +362,9 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                     L.tmp[g] = L.p[g] + L.P.relxp * (ppv - 0.0);
             } else if (INL && field == 0) {
                 /* the merged relaxation: u + du formed here, once */
-                L.uc[g] = L.u[g] + (double)t[p];
+                L.uc[g] = pending_value(L.u[g], t[p], L.pset && (L.flags[g] & MGF_F_SPU), 0.0);
             } else if (INL && field == 1) {
-                L.vc[g] = L.v[g] + (double)t[p];
+                L.vc[g] = pending_value(L.v[g], t[p], L.pset && (L.flags[g] & MGF_F_SPV), 0.0);
             } else {
                 dst[g] = t[p];
             }
@@ -454,7 +456,8 @@ __global__ void __launch_bounds__(BX * BY) k_jac32(const DevLevel *Ls, int field
 }
 
 /* the result of nl sweep launches (its parity names the buffer holding d)
- * over the swept range: sc += d in FP64; pp = d in FP64; for u and v the
+ * over the swept range: sc += d in FP64 (penalised cells set to sc_obs:
+ * pending_value; nothing without sweeps); pp = d in FP64; for u and v the
  * correction is left pending in du32 / dv32 (zero without sweeps) */
 __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int field, int nl)
 {
@@ -471,7 +474,8 @@ __global__ void __launch_bounds__(BX * BY) k_apply_mx(const DevLevel *Ls, int fi
     else if (field == 2)
         phi[ij] = nl > 0 ? (double)d[ij] : 0.0;
     else
-        phi[ij] = phi[ij] + (double)d[ij];
+        phi[ij] = nl > 0 ? pending_value(phi[ij], d[ij], L.pset && (L.flags[ij] & MGF_F_SPSC), L.P.sc_obs)
+                         : phi[ij];
 }
 
 /* ====================================================================== */

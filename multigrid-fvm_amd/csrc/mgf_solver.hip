@@ -252,6 +252,7 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
         I.init_state[ig] = (real_t *)take(4 * nb);
         L.partial = (double *)take((size_t)L.nblk * sizeof(double));
         L.isolve = c->isolve;
+        L.pset = c->nswpm >= 1;
         if (s->opt.precision == MGF_PREC_MIXED) {
             float **a32[11] = {&L.aw32, &L.ae32, &L.as32, &L.an32, &L.iap32, &L.r32,
                                &L.da32, &L.db32, &L.du32, &L.dv32, &L.ds32};

@@ -180,6 +180,19 @@ __device__ __forceinline__ Coef coefsc_at(const DevLevelG &L, const FX &f, int i
     return coefsc_vals(L, ij, fl, f.X(ij - 1), f.X(ij), f.Y(ijs), f.Y(ij));
 }
 
+/* Mixed precision: the value an FP32 correction d leaves at a point of its
+ * swept range.  A penalised point (1e30 added to its diagonal: flag pen) is
+ * set rather than corrected.  Its FP64 Gauss-Seidel value is
+ * (su + sum a_nb phi_nb) / (ap + 1e30), which is `target` to ~1e-30
+ * absolute: 0 for u and v faces, sc_obs for sc cells.  phi + d would keep
+ * the FP32 rounding of d ~ -phi, ~1e-7 |phi|; that decays only by 1e-7 per
+ * relaxation, and a fluid cell whose four faces are all penalised turns its
+ * divergence into p' through 1/ap ~ 1e33. */
+__device__ __forceinline__ real_t pending_value(real_t phi, float d, bool pen, real_t target)
+{
+    return pen ? target : phi + (real_t)d;
+}
+
 #define RANGE2D(i0, i1, j0, j1)                                   \
     const int i = (i0) + (int)(T.x * BX + threadIdx.x);           \
     const int j = (j0) + (int)(T.y * BY + threadIdx.y);           \

@@ -266,3 +266,123 @@ def test_merged_relaxation_boundary_variants(tmp_path, names):
         o.close()
     split.close()
     bt.close()
+
+
+def _random_batch(seed):
+    """Six seeded problems sharing one smoother and sweep counts (a batch's
+    requirement), each with its own size, depth, obstacle level (coarsened
+    obstacles below it when ig_obs < ngrid), Bernoulli mask, Reynolds number
+    and outflow boundary choice."""
+    rng = np.random.default_rng(seed)
+    isolve = int(rng.integers(1, 3))
+    nswpm, nswpp = int(rng.integers(1, 4)), int(rng.integers(1, 7))
+    probs = []
+    for k in range(6):
+        n = int(rng.choice([32, 64]))
+        ng = int(rng.integers(2, 5))
+        ig_obs = int(rng.integers(1, ng + 1))
+        base = n >> (ig_obs - 1)
+        mask = random_mask(base, float(rng.uniform(0.75, 0.9)), int(rng.integers(1, 10**6)))
+        ibc = (1, 4, 1, 1) if rng.uniform() < 0.3 else (1, 1, 1, 1)
+        probs.append(Problem(nfx=n, nfy=n, ngrid=ng, ig_obs=ig_obs, isolve=isolve, nswpm=nswpm,
+                             nswpp=nswpp, nvcycle=10, convergence=1e-30,
+                             amuf=float(rng.choice([0.1, 0.05, 0.02])), ibc=ibc,
+                             obstacles=mask))
+    return probs
+
+
+@pytest.mark.parametrize("seed", [9, 38, 4, 29, 50])
+def test_random_batches_bitwise(tmp_path, seed):
+    """Seeded random batches in FP64: every member's cycle count, history and
+    every state array of every level as the oracle's.  Seeds 9 and 38 draw
+    Jacobi, 4, 29 and 50 red-black; all five mix depths 2-4, coarsened
+    obstacles and outflow members, and stay finite over their 10 cycles."""
+    import mgfvm
+    probs = _random_batch(seed)
+    p0 = probs[0]
+    _say(f"[gpu] random batch seed {seed}: isolve {p0.isolve}, sweeps {p0.nswpm}/{p0.nswpp}, "
+         + ", ".join(f"{p.nfx}^2 ng{p.ngrid} obs{p.ig_obs} ibc{p.ibc[1]}" for p in probs))
+    pairs = [p.write(str(tmp_path / f"r{k}")) for k, p in enumerate(probs)]
+    bt = mgfvm.Batch(pairs)
+    ncyc, conv = bt.solve()
+    for k, (p, (pin, ppat)) in enumerate(zip(probs, pairs)):
+        o = orc.Oracle(pin, ppat)
+        ho, no, co = o.solve(p.nvcycle, p.convergence)
+        assert np.all(np.isfinite(ho)), (k, "pick parameters that stay finite")
+        assert ncyc[k] == no and conv[k] == co, k
+        np.testing.assert_allclose(bt.history(k), ho, rtol=1e-12, atol=0.0)
+        for ig in range(o.ngrid):
+            for n in STATE:
+                g, r = bt.get(k, ig, n), o.get(ig, n)
+                assert np.array_equal(g.view(np.uint64), r.view(np.uint64)), (k, ig, n)
+        o.close()
+    bt.close()
+
+
+def _connected_cells(o):
+    """Level-0 cells reachable from a fluid cell of the inlet column through
+    open (unpenalised) faces.  The others are fluid pockets cut off by
+    obstacles, and obstacle cells: their p' rows are ~1e-33 times the
+    identity, so their p is not determined by the equations (it differs
+    between any two arithmetics, FP64 runs included)."""
+    tag, spu, spv = o.get(0, "tag"), o.get(0, "spu"), o.get(0, "spv")
+    ny, nx = tag.shape
+    seen = np.zeros((ny, nx), bool)
+    stack = [(j, 1) for j in range(1, ny - 1) if tag[j, 1] != 0]
+    for j, i in stack:
+        seen[j, i] = True
+    while stack:
+        j, i = stack.pop()
+        # neighbours through the u faces (i-1 | i, i | i+1) and v faces
+        for jj, ii, open_ in ((j, i - 1, spu[j, i - 1] < 1e10), (j, i + 1, spu[j, i] < 1e10),
+                              (j - 1, i, spv[j - 1, i] < 1e10), (j + 1, i, spv[j, i] < 1e10)):
+            if (1 <= ii <= nx - 2 and 1 <= jj <= ny - 2 and open_ and not seen[jj, ii]
+                    and tag[jj, ii] != 0):
+                seen[jj, ii] = True
+                stack.append((jj, ii))
+    return seen
+
+
+@pytest.mark.parametrize("seed", [4, 29, 50])
+def test_random_batches_mixed(tmp_path, seed):
+    """The red-black random batches in mixed precision: bit for bit as the
+    field-by-field split-assembly batch (members with an outflow boundary take
+    the field-by-field sequence in both), and within the mixed tolerance of
+    the FP64 oracle: u, v, sc everywhere, p on the cells connected to the
+    flow, the residual history to 1e-6 of its largest entry.  Random masks
+    make fluid cells enclosed by obstacles and start penalised faces at the
+    guess, which the FP32 corrections must not leave at their rounding."""
+    import mgfvm
+    probs = _random_batch(seed)
+    _say(f"[gpu] random mixed batch seed {seed}")
+    pairs = [p.write(str(tmp_path / f"m{k}")) for k, p in enumerate(probs)]
+    split = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, split_assembly=True)
+    ns, cs = split.solve()
+    bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED)
+    nb, cb = bt.solve()
+    assert (list(nb), list(cb)) == (list(ns), list(cs))
+    for k, (p, (pin, ppat)) in enumerate(zip(probs, pairs)):
+        assert np.array_equal(split.history(k), bt.history(k)), k
+        for ig in range(p.ngrid):
+            for name in STATE:
+                assert np.array_equal(split.get(k, ig, name).view(np.uint64),
+                                      bt.get(k, ig, name).view(np.uint64)), (k, ig, name)
+        o = orc.Oracle(pin, ppat)
+        ho, no, co = o.solve(p.nvcycle, p.convergence)
+        assert (nb[k], cb[k]) == (no, co), k
+        dev = {name: np.abs(bt.get(k, 0, name) - o.get(0, name)).max()
+               / max(np.abs(o.get(0, name)).max(), 1e-300) for name in ("u", "v", "sc")}
+        # p where the equations determine it (cells connected to the flow)
+        conn = _connected_cells(o)
+        gp, rp = bt.get(k, 0, "p")[conn], o.get(0, "p")[conn]
+        dev["p"] = np.abs(gp - rp).max() / max(np.abs(rp).max(), 1e-300)
+        hdev = np.abs(bt.history(k) - ho).max() / np.abs(ho).max()
+        _say(f"  member {k} ({p.nfx}^2 ng{p.ngrid} obs{p.ig_obs} ibc{p.ibc[1]}): field dev "
+             + " ".join(f"{n} {v:.2e}" for n, v in dev.items()) + f", history dev {hdev:.2e}, "
+             + f"{int((o.get(0, 'tag')[1:-1, 1:-1] != 0).sum() - conn.sum())} cut-off fluid cells")
+        for name, v in dev.items():
+            assert v < MIXED_FIELD_RTOL, (k, name, v)
+        assert hdev < 1e-6, (k, hdev)
+        o.close()
+    split.close()
+    bt.close()
