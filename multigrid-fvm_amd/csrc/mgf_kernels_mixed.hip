@@ -294,7 +294,8 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     double apk[ASM ? NPT : 1];
     /* per slot: distance to the region's edge (bits 0-7), colour (bit 8),
      * inside the swept range (bit 9), a copy origin whose updated p is
-     * stashed (bit 10) */
+     * stashed (bit 10), a penalised u / v face of the merged launch
+     * (bit 11) */
     unsigned meta[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
@@ -316,12 +317,17 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                         if (d >= H) mg[(r - H) * BX + (c - H)] = mag;
                     } else {
                         pt[k] = asm_pt<INL>(L, field, gi, gj, apk[ASM ? k : 0]);
+                        /* the merged launch: a penalised u / v face (the
+                         * assembly has loaded its flag already) */
+                        if (INL && field != 3)
+                            m |= (unsigned)((L.flags[g] &
+                                             (field == 0 ? MGF_F_SPU : MGF_F_SPV)) != 0) << 11;
                     }
                 } else {
                     pt[k] = load_pt(L, g);
                 }
                 if (src) val = src[g];
-                m = (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
+                m |= (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
                 if (stash && d >= H && (L.flags[g] & MGF_F_PORIG)) m |= 1u << 10;
             }
             t[p] = val;
@@ -362,9 +368,9 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                     L.tmp[g] = L.p[g] + L.P.relxp * (ppv - 0.0);
             } else if (INL && field == 0) {
                 /* the merged relaxation: u + du formed here, once */
-                L.uc[g] = pending_value(L.u[g], t[p], L.pset && (L.flags[g] & MGF_F_SPU), 0.0);
+                L.uc[g] = pending_value(L.u[g], t[p], L.pset && (m & (1u << 11)), 0.0);
             } else if (INL && field == 1) {
-                L.vc[g] = pending_value(L.v[g], t[p], L.pset && (L.flags[g] & MGF_F_SPV), 0.0);
+                L.vc[g] = pending_value(L.v[g], t[p], L.pset && (m & (1u << 11)), 0.0);
             } else {
                 dst[g] = t[p];
             }
