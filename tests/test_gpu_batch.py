@@ -291,20 +291,62 @@ def _random_batch(seed):
     return probs
 
 
+def _random_batch_io(seed):
+    """Discrete inlets and outlets given on a random level (ig_inlt,
+    ig_outlt), and an obstacle offset (ibeg_obs)."""
+    rng = np.random.default_rng(seed)
+    isolve = int(rng.integers(1, 3))
+    nswpm, nswpp = int(rng.integers(1, 4)), int(rng.integers(1, 7))
+    probs = []
+    for k in range(6):
+        n = int(rng.choice([32, 64]))
+        ng = int(rng.integers(2, 5))
+        ig_obs = int(rng.integers(1, ng + 1))
+        ibeg = int(rng.integers(0, 3))
+        b = n >> (ig_obs - 1)
+        blocked = rng.uniform(size=(b, b)) < 0.15
+        rects = [(i + 1, i + 1, j + 1, j + 1) for i in range(1, b - 1 - ibeg) for j in range(b)
+                 if blocked[i, j]]
+
+        def ports(ig):
+            rows = n >> (ig - 1)
+            out, j = [], 1
+            while j <= rows and len(out) < 3:
+                jf = j + int(rng.integers(0, max(1, rows // 4)))
+                jl = min(rows, jf + int(rng.integers(0, max(1, rows // 4))))
+                if jf > rows:
+                    break
+                out.append((jf, jl, float(rng.uniform(0.3, 1.5)), float(rng.uniform(-0.1, 0.1)),
+                            float(rng.uniform(0.0, 2.0))))
+                j = jl + 2
+            return out
+
+        ig_in, ig_out = int(rng.integers(1, ng + 1)), int(rng.integers(1, ng + 1))
+        probs.append(Problem(nfx=n, nfy=n, ngrid=ng, ig_obs=ig_obs, ibeg_obs=ibeg,
+                             ig_inlt=ig_in, ig_outlt=ig_out, isolve=isolve, nswpm=nswpm,
+                             nswpp=nswpp, nvcycle=8, convergence=1e-30,
+                             amuf=float(rng.choice([0.1, 0.05])), inlets=ports(ig_in),
+                             outlets=ports(ig_out), obstacles=rects))
+    return probs
+
+
 def _family(name, seed):
-    return (_random_batch if name == "mask" else _random_batch_bc)(seed)
+    return {"mask": _random_batch, "bc": _random_batch_bc, "io": _random_batch_io}[name](seed)
 
 
 @pytest.mark.parametrize("family,seed", [("mask", 9), ("mask", 38), ("mask", 4), ("mask", 29),
                                          ("mask", 50), ("bc", 6), ("bc", 35), ("bc", 31),
-                                         ("bc", 33), ("bc", 70)])
+                                         ("bc", 33), ("bc", 70), ("io", 9), ("io", 23),
+                                         ("io", 10), ("io", 15), ("io", 29)])
 def test_random_batches_bitwise(tmp_path, family, seed):
     """Seeded random batches in FP64: every member's cycle count, history and
     every state array of every level as the oracle's.  "mask" seeds 9 and 38
     draw Jacobi, 4, 29 and 50 red-black; all mix depths 2-4, coarsened
     obstacles and outflow members.  "bc" seeds 6 and 35 draw Jacobi, 31, 33
     and 70 red-black, each with at least three of the four boundary sets on
-    rectangular grids.  Every state array of every member stays finite (a
+    rectangular grids.  "io" seeds 9 and 23 draw Jacobi, 10, 15 and 29
+    red-black, with discrete inlets and outlets on random levels and obstacle
+    offsets.  Every state array of every member stays finite (a
     NaN's bits differ between the host's and the GPU's arithmetic: seed 26,
     whose scalar diverges on a periodic member, is left out for that)."""
     import mgfvm
@@ -356,7 +398,8 @@ def _connected_cells(o):
 
 
 @pytest.mark.parametrize("family,seed", [("mask", 4), ("mask", 29), ("mask", 50), ("bc", 31),
-                                         ("bc", 33), ("bc", 70)])
+                                         ("bc", 33), ("bc", 70), ("io", 10), ("io", 15),
+                                         ("io", 29)])
 def test_random_batches_mixed(tmp_path, family, seed):
     """The red-black random batches in mixed precision: bit for bit as the
     field-by-field split-assembly batch (members with an outflow boundary take
