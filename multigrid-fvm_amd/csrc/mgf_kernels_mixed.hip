@@ -218,6 +218,16 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #ifndef MGF_UVS_TILE_ROWS
 #define MGF_UVS_TILE_ROWS 32
 #endif
+/* tile rows and threads of the merged launch on levels of at most 64 x 64:
+ * 16-row tiles (4 slots per thread instead of 6) shorten each workgroup's
+ * chain of dependent loads, which is what such a launch waits for (0.7% on
+ * the batch; 1024 threads, or 32 rows with 1024, were slower) */
+#ifndef MGF_UVS_SMALL_ROWS
+#define MGF_UVS_SMALL_ROWS 16
+#endif
+#ifndef MGF_UVS_SMALL_THREADS
+#define MGF_UVS_SMALL_THREADS 512
+#endif
 /* tile rows of the other assembling launches */
 #ifndef MGF_ASM_TILE_ROWS
 #define MGF_ASM_TILE_ROWS MGF_SWEEP_TILE_ROWS
@@ -421,7 +431,7 @@ __global__ void __launch_bounds__(NT) k_rb32x(const DevLevel *Ls, int field_rt, 
  * correction stays pending in ds32 until the update.  The grid is the sc
  * tile grid with three logical z-slices per member (field = z % 3); tiles
  * past u's or v's smaller range hold no unknowns and write nothing. */
-template <int S, int NT>
+template <int S, int NT, int TYU = MGF_UVS_TILE_ROWS>
 __global__ void __launch_bounds__(NT) k_rb32x_uvs(const DevLevel *Ls)
 {
     const TileId T3 = tile_id();
@@ -429,15 +439,15 @@ __global__ void __launch_bounds__(NT) k_rb32x_uvs(const DevLevel *Ls)
     const int f = T3.z - 3 * T.z;
     const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[T.z];
     if (L.ctl->done) return;
-    __shared__ float t[RbShape<S, MGF_UVS_TILE_ROWS>::NR];
+    __shared__ float t[RbShape<S, TYU>::NR];
     __shared__ double mg[BX];
     __shared__ double rows[1];
     if (f == 0)
-        rb32x_tile<S, NT, MODE_ASM_INL, 0, MGF_UVS_TILE_ROWS>(L, T, 0, 0, 1, t, mg, rows);
+        rb32x_tile<S, NT, MODE_ASM_INL, 0, TYU>(L, T, 0, 0, 1, t, mg, rows);
     else if (f == 1)
-        rb32x_tile<S, NT, MODE_ASM_INL, 1, MGF_UVS_TILE_ROWS>(L, T, 1, 0, 1, t, mg, rows);
+        rb32x_tile<S, NT, MODE_ASM_INL, 1, TYU>(L, T, 1, 0, 1, t, mg, rows);
     else
-        rb32x_tile<S, NT, MODE_ASM_INL, 3, MGF_UVS_TILE_ROWS>(L, T, 3, 0, 1, t, mg, rows);
+        rb32x_tile<S, NT, MODE_ASM_INL, 3, TYU>(L, T, 3, 0, 1, t, mg, rows);
 }
 
 /* one FP32 Jacobi sweep of A d = r (wj = 1, V6:1241) */
@@ -725,6 +735,15 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
 template <int S>
 static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
 {
+    if (small_group(G)) {
+        /* levels of at most 64 x 64: latency-bound, shaped by their own
+         * tile rows and threads */
+        const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_UVS_SMALL_ROWS);
+        const dim3 g(g0.x, g0.y, 3 * g0.z);
+        hipLaunchKernelGGL((k_rb32x_uvs<S, MGF_UVS_SMALL_THREADS, MGF_UVS_SMALL_ROWS>), g,
+                           dim3(BX, MGF_UVS_SMALL_THREADS / BX), 0, st, G.d);
+        return;
+    }
     const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_UVS_TILE_ROWS);
     const dim3 g(g0.x, g0.y, 3 * g0.z);
     if (nt == 1024)
