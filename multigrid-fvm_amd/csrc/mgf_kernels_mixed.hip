@@ -218,10 +218,11 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #ifndef MGF_UVS_TILE_ROWS
 #define MGF_UVS_TILE_ROWS 32
 #endif
-/* tile rows and threads of the merged launch on levels of at most 64 x 64:
- * 16-row tiles (4 slots per thread instead of 6) shorten each workgroup's
- * chain of dependent loads, which is what such a launch waits for (0.7% on
- * the batch; 1024 threads, or 32 rows with 1024, were slower) */
+/* tile rows and threads of the merged launch on small levels (at most
+ * MGF_UVS_SMALL_MAX): 16-row tiles (4 slots per thread instead of 6) shorten
+ * each workgroup's chain of dependent loads, which is what such a launch
+ * waits for (1024 threads, or 32 rows with 1024, were slower; 8 rows no
+ * better) */
 #ifndef MGF_UVS_SMALL_ROWS
 #define MGF_UVS_SMALL_ROWS 16
 #endif
@@ -633,6 +634,21 @@ static bool small_group(const DevGroup &G)
     return G.npx - 2 <= BX && G.npy - 2 <= 4 * MGF_SWEEP_TILE_ROWS;
 }
 
+/* levels of at most n x n interior points */
+static bool level_at_most(const DevGroup &G, int n)
+{
+    return G.npx - 2 <= n && G.npy - 2 <= n;
+}
+/* the merged relaxation takes the small-level shapes (16-row tiles; for p'
+ * 1024 threads) up to these sizes: on 128 x 128 levels too they shorten the
+ * latency-bound workgroups (2.5% on the batch against 64; 256 was slower) */
+#ifndef MGF_UVS_SMALL_MAX
+#define MGF_UVS_SMALL_MAX 128
+#endif
+#ifndef MGF_PP_SMALL_MAX
+#define MGF_PP_SMALL_MAX 128
+#endif
+
 template <int S>
 static void launch_small_s(const DevGroup &G, int field, int last, int nx, int ny,
                            hipStream_t st)
@@ -735,9 +751,9 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
 template <int S>
 static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
 {
-    if (small_group(G)) {
-        /* levels of at most 64 x 64: latency-bound, shaped by their own
-         * tile rows and threads */
+    if (level_at_most(G, MGF_UVS_SMALL_MAX)) {
+        /* small levels: latency-bound, shaped by their own tile rows and
+         * threads */
         const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_UVS_SMALL_ROWS);
         const dim3 g(g0.x, g0.y, 3 * g0.z);
         hipLaunchKernelGGL((k_rb32x_uvs<S, MGF_UVS_SMALL_THREADS, MGF_UVS_SMALL_ROWS>), g,
@@ -776,7 +792,7 @@ static void launch_pp_merged_s(const DevGroup &G, hipStream_t st)
 {
     int nx, ny;
     sweep_extents(G, 2, nx, ny);
-    if (small_group(G)) {
+    if (level_at_most(G, MGF_PP_SMALL_MAX)) {
         const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SMALL_TILE_ROWS);
         hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM_CORR, 2, MGF_SMALL_TILE_ROWS>), g,
                            dim3(BX, 16), 0, st, G.d, 2, 0, 3);
