@@ -48,11 +48,11 @@ sys.path.insert(0, os.path.join(ROOT, "multigrid-fvm_amd"))
 METRIC = ("converged solves/s at FP64 residual 1e-6, batched-mixed vs plain GPU; "
           "% HBM peak")
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the roofline kernel (k_rb32x_uvs<2,512> over the 84
+# HBM bytes per launch of the roofline kernel (k_rb32x_uvs<2,512,32> over the 84
 # 512^2 members): rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # `bench.py --roofline-only --reps 10`, each scaled by what the counter reads
 # for a 1 GiB device copy (profiles/traffic_relax_uvs_512x84.md)
-TRAFFIC = 2240.2e6
+TRAFFIC = 2241.1e6
 WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
             "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
             "each obstacle moved by a seeded quarter/half/full cell), Re in "
@@ -102,7 +102,7 @@ def time_reference_cycle(prob, workdir, target_s=4.0):
 
 def kernel_roofline(probs, pairs, reps, cycles=None):
     """The dominant kernel of a batched mixed V-cycle is the launch that
-    relaxes u, v and sc together (k_rb32x_uvs<2,512>: for each field, FP64
+    relaxes u, v and sc together (k_rb32x_uvs<2,512,32>: for each field, FP64
     assembly of its equation on the tile and halo, two FP32 red-black sweeps,
     FP32 correction out).  Its mean duration is timed with HIP events on the
     batch's own stream over every member of the largest grid size at level 0,
@@ -128,7 +128,7 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
     ncell = nmax * nmax * len(big_pairs)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC,
-            "kernel": f"k_rb32x_uvs<2,512>: u, v and sc relaxed in one launch (FP64 assembly + "
+            "kernel": f"k_rb32x_uvs<2,512,32>: u, v and sc relaxed in one launch (FP64 assembly + "
                       f"2 FP32 red-black sweeps each) over the {len(big_pairs)} {nmax}^2 "
                       f"members, level 0",
             "mean_us": round(us, 2), "bytes_per_launch": nbytes,
