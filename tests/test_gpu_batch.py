@@ -469,3 +469,26 @@ def _random_batch_bc(seed):
                              amuf=float(rng.choice([0.1, 0.05])), ibc=bcs[int(rng.integers(0, 4))],
                              sc_obs=float(rng.choice([0.0, 0.5])), obstacles=rects))
     return probs
+
+
+def test_random_problems_mixed_cycle_counts_equal_fp64(tmp_path):
+    """540 seeded random problems (the three families above, seeds 200-229,
+    red-black 2/6) run to sum|div| < 1e-6 or 400 cycles as one FP64 batch
+    and one mixed batch: every problem stops after the same number of cycles
+    with the same converged flag (the FP64 batch is the oracle bit for bit,
+    by the tests above)."""
+    import mgfvm
+    probs = [p.replace(isolve=2, nswpm=2, nswpp=6, nvcycle=400, convergence=1e-6)
+             for fam in ("mask", "bc", "io") for seed in range(200, 230)
+             for p in _family(fam, seed)]
+    pairs = [p.write(str(tmp_path / f"c{k}")) for k, p in enumerate(probs)]
+    _say(f"[gpu] {len(pairs)} random problems to convergence, FP64 and mixed")
+    f64 = mgfvm.Batch(pairs)
+    n64, c64 = f64.solve()
+    f64.close()
+    mx = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED)
+    nmx, cmx = mx.solve()
+    mx.close()
+    assert sum(c64) >= 60, "the set should hold converging problems"
+    diff = [k for k in range(len(pairs)) if (n64[k], c64[k]) != (nmx[k], cmx[k])]
+    assert not diff, [(k, n64[k], nmx[k]) for k in diff[:10]]
