@@ -144,6 +144,7 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
 int mgf_field_array(mgf_solver *s, int b, int ig, const char *name, real_t **ptr, size_t *n,
                     hipStream_t *st);
 const mgf_problem *mgf_instance_problem(mgf_solver *s, int b);
+const mgf_options *mgf_solver_options(mgf_solver *s);
 int mgf_set_error(mgf_solver *s, int code, const char *msg);
 
 /* launch wrappers (mgf_kernels.hip) */

@@ -1168,6 +1168,11 @@ const mgf_problem *mgf_instance_problem(mgf_solver *s, int b)
     return (s && b >= 0 && b < s->nb) ? &s->inst[b].plan->prob : NULL;
 }
 
+const mgf_options *mgf_solver_options(mgf_solver *s)
+{
+    return s ? &s->opt : NULL;
+}
+
 int mgf_set_error(mgf_solver *s, int code, const char *msg)
 {
     return set_err(s, code, "%s", msg);
