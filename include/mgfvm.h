@@ -240,6 +240,14 @@ int mgf_get_field(mgf_solver *s, int ig, const char *name, double *out);
 int mgf_set_field(mgf_solver *s, int ig, const char *name, const double *in);
 int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
                    double *pp, double *sc);
+/* Replace problem b's inlet u profile (nfy values on the level-0 west face,
+ * rows 1..nfy, as mgf_problem.inlet_u; NULL: the InputData inlet) and reset
+ * problem b to the initial state built with it, as mgf_create would have
+ * built it (init and rest_geom's inlet values, V6:546-676, 1440-1446).
+ * b = 0 for a single-problem solver.  With a plug east boundary the caller
+ * keeps the profile's mean equal to the east u.  The sensitivity library
+ * takes the profile at creation only (MGF_ERR_UNSUPPORTED here). */
+int mgf_set_inlet_profile(mgf_solver *s, int b, const double *inlet_u);
 
 /* ---- measurement ------------------------------------------------------------ */
 

@@ -798,25 +798,90 @@ static int read_ctl(mgf_solver *s)
  * and boundary values), pp and the tau terms (zero), and the momentum
  * diagonals (1e30; their ghost entries are never rewritten).  The other
  * arrays are scratch that every step rewrites before it reads them. */
+static int reset_instance(mgf_solver *s, Inst &I)
+{
+    for (int ig = 0; ig < I.ng; ++ig) {
+        DevLevel &L = I.lv[ig];
+        size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(real_t);
+        real_t *dst4[4] = {L.u, L.v, L.p, L.sc};
+        for (int k = 0; k < 4; ++k)
+            HIPCHK(s, hipMemcpyAsync(dst4[k], I.init_state[ig] + k * n, nb,
+                                     hipMemcpyDeviceToDevice, s->st));
+        mgf_launch_fill(L.pp, 0.0, n, s->st);
+        mgf_launch_fill(L.resu, 0.0, n, s->st);
+        mgf_launch_fill(L.resv, 0.0, n, s->st);
+        mgf_launch_fill(L.ressc, 0.0, n, s->st);
+        mgf_launch_fill(L.apu, 1.0e30, n, s->st);
+        mgf_launch_fill(L.apv, 1.0e30, n, s->st);
+    }
+    return MGF_OK;
+}
+
 extern "C" int mgf_reset(mgf_solver *s)
 {
     if (!s) return set_err(s, MGF_ERR_ARG, "null solver");
-    for (Inst &I : s->inst)
-        for (int ig = 0; ig < I.ng; ++ig) {
-            DevLevel &L = I.lv[ig];
-            size_t n = (size_t)L.npx * L.npy, nb = n * sizeof(real_t);
-            real_t *dst4[4] = {L.u, L.v, L.p, L.sc};
-            for (int k = 0; k < 4; ++k)
-                HIPCHK(s, hipMemcpyAsync(dst4[k], I.init_state[ig] + k * n, nb,
-                                         hipMemcpyDeviceToDevice, s->st));
-            mgf_launch_fill(L.pp, 0.0, n, s->st);
-            mgf_launch_fill(L.resu, 0.0, n, s->st);
-            mgf_launch_fill(L.resv, 0.0, n, s->st);
-            mgf_launch_fill(L.ressc, 0.0, n, s->st);
-            mgf_launch_fill(L.apu, 1.0e30, n, s->st);
-            mgf_launch_fill(L.apv, 1.0e30, n, s->st);
-        }
+    for (Inst &I : s->inst) {
+        int rc = reset_instance(s, I);
+        if (rc) return rc;
+    }
     return check_launch(s);
+}
+
+/* A new inlet profile for problem b: its initial state is built again from
+ * its problem with inlet_u replaced (init and rest_geom, V6:546-676,
+ * 1391-1473: the level-0 west face and the coarse inlet values), and the
+ * problem is reset to it, as if the solver had been created with it. */
+extern "C" int mgf_set_inlet_profile(mgf_solver *s, int b, const double *inlet_u)
+{
+    if (!s || b < 0 || b >= s->nb) return set_err(s, MGF_ERR_ARG, "bad solver or problem index");
+#ifdef MGF_TANGENT
+    (void)inlet_u;
+    return set_err(s, MGF_ERR_UNSUPPORTED,
+                   "the sensitivity library takes the inlet profile (with its seed) at creation");
+#else
+    Inst &I = s->inst[b];
+    mgf_problem q;
+    mgf_problem_copy(&I.plan->prob, &q);
+    free((void *)q.inlet_u);
+    q.inlet_u = NULL;
+    if (inlet_u) {
+        double *c = (double *)malloc(sizeof(double) * (size_t)q.nfy);
+        if (!c) {
+            mgf_problem_release(&q);
+            return set_err(s, MGF_ERR_LIMIT, "out of host memory");
+        }
+        memcpy(c, inlet_u, sizeof(double) * (size_t)q.nfy);
+        q.inlet_u = c;
+    }
+    mgf_plan *np = NULL;
+    char e[256] = "";
+    int rc = mgf_plan_build(&q, s->opt.strict_limits, &np, e, sizeof e);
+    mgf_problem_release(&q);
+    if (rc) return set_err(s, rc, "%s", e);
+    for (int ig = 0; ig < I.ng; ++ig) {
+        const mgf_plan_level *l = &np->lv[ig];
+        const size_t n = (size_t)l->npx * l->npy;
+        const double *src4[4] = {l->u, l->v, l->p, l->sc};
+        for (int k = 0; k < 4; ++k) {
+            const hipError_t he = to_device(I.init_state[ig] + k * n, src4[k], n, s->st);
+            if (he != hipSuccess) {
+                (void)hipStreamSynchronize(s->st);
+                mgf_plan_free(np);
+                return set_err(s, MGF_ERR_DEVICE, "%s", hipGetErrorString(he));
+            }
+        }
+    }
+    /* the pageable host arrays must stay alive until the copies finish */
+    const hipError_t he = hipStreamSynchronize(s->st);
+    if (he != hipSuccess) {
+        mgf_plan_free(np);
+        return set_err(s, MGF_ERR_DEVICE, "%s", hipGetErrorString(he));
+    }
+    mgf_plan_free(I.plan);
+    I.plan = np;
+    if ((rc = reset_instance(s, I))) return rc;
+    return check_launch(s);
+#endif
 }
 
 extern "C" int mgf_synchronize(mgf_solver *s)

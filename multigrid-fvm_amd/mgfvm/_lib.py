@@ -24,7 +24,7 @@ EXPORTS = [
     "mgf_time_kernel", "mgf_reset", "mgf_synchronize", "mgf_batch_create", "mgf_batch_size",
     "mgf_batch_num_levels", "mgf_batch_level_dims", "mgf_batch_solve", "mgf_batch_history",
     "mgf_batch_get_field", "mgf_pattern_bernoulli", "mgf_pattern_staggered",
-    "mgf_pattern_particles",
+    "mgf_pattern_particles", "mgf_set_inlet_profile",
 ]
 
 PREC_FP64 = 0
@@ -104,6 +104,7 @@ def lib():
         L.mgf_time_kernel.argtypes = [vp, ip, ip, ip, P_d, P_d]
         L.mgf_reset.argtypes = [vp]
         L.mgf_synchronize.argtypes = [vp]
+        L.mgf_set_inlet_profile.argtypes = [vp, ip, P_d]
         PP = ctypes.POINTER(ProblemStruct)
         L.mgf_batch_create.argtypes = [ctypes.POINTER(PP), ip, ctypes.POINTER(Options),
                                        ctypes.POINTER(vp)]
@@ -138,6 +139,16 @@ def read_problem(inputdata: str, pattern: str):
 
 def free_problem(p):
     lib().mgf_problem_free(p)
+
+
+def _profile_ptr(inlet_u, nfy: int):
+    """A pointer to nfy inlet values (it keeps the array alive), or NULL."""
+    if inlet_u is None:
+        return ctypes.POINTER(ctypes.c_double)()
+    a = np.ascontiguousarray(inlet_u, dtype=np.float64)
+    if a.size != nfy:
+        raise ValueError(f"inlet profile needs {nfy} values")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 class InletProfile:
@@ -244,6 +255,12 @@ class Solver:
         """Back to the initial state, on the device (asynchronous)."""
         self._check(lib().mgf_reset(self.h))
 
+    def set_inlet_profile(self, inlet_u=None):
+        """A new inlet u profile (nfy values; None: the InputData inlet), and
+        the state reset to the initial state built with it."""
+        self._check(lib().mgf_set_inlet_profile(self.h, 0,
+                                                _profile_ptr(inlet_u, self.dims(0)[1] - 2)))
+
     def synchronize(self):
         self._check(lib().mgf_synchronize(self.h))
 
@@ -328,6 +345,12 @@ class Batch:
 
     def reset(self):
         self._check(lib().mgf_reset(self.h))
+
+    def set_inlet_profile(self, b: int, inlet_u=None):
+        """Problem b's inlet u profile (None: its InputData inlet); problem b
+        is reset to the initial state built with it."""
+        self._check(lib().mgf_set_inlet_profile(self.h, b,
+                                                _profile_ptr(inlet_u, self.dims(b, 0)[1] - 2)))
 
     def synchronize(self):
         self._check(lib().mgf_synchronize(self.h))

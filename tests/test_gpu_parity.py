@@ -184,3 +184,49 @@ def test_reset_reproduces_fresh_solve(tmp_path):
     for ig in range(a.ngrid):
         for n in STATE:
             assert np.array_equal(a.get(ig, n).view(np.uint64), b.get(ig, n).view(np.uint64)), (ig, n)
+
+
+def test_set_inlet_profile_equals_creation_with_it(tmp_path):
+    """mgf_set_inlet_profile: a solver given an inlet profile after creation
+    solves to the bits of one created with it, on every level; NULL brings
+    back the InputData inlet; a batch member takes its own profile."""
+    import mgfvm
+    from mgfvm.inputs import Problem, random_mask
+    nf = 32
+    j = (np.arange(nf) + 0.5) / nf
+    prof = 1.0 + 0.3 * np.cos(np.pi * j) + 0.1 * np.sin(5.0 * np.pi * j)
+    prob = Problem(nfx=nf, nfy=nf, ngrid=3, ig_obs=3, isolve=2, nvcycle=40, nswpp=6,
+                   bc_xp=(float(prof.mean()), 0.0, 0.0, 0.0), obstacles=random_mask(8, 0.85, 3))
+    pin, ppat = prob.write(str(tmp_path / "ip"))
+    _say("[gpu] set_inlet_profile")
+
+    def state(s):
+        return {(ig, n): s.get(ig, n).copy() for ig in range(prob.ngrid) for n in STATE}
+
+    def same(x, y):
+        return all(np.array_equal(x[k].view(np.uint64), y[k].view(np.uint64)) for k in x)
+
+    a = mgfvm.Solver(pin, ppat, inlet_u=prof)
+    ha = a.solve()
+    b = mgfvm.Solver(pin, ppat)
+    b.solve()                                   # a solve with the InputData inlet first
+    b.set_inlet_profile(prof)
+    hb = b.solve()
+    assert (ha[1], ha[2]) == (hb[1], hb[2]) and np.array_equal(ha[0], hb[0])
+    assert same(state(a), state(b))
+    c = mgfvm.Solver(pin, ppat)
+    hc = c.solve()
+    b.set_inlet_profile(None)
+    hb2 = b.solve()
+    assert (hc[1], hc[2]) == (hb2[1], hb2[2]) and np.array_equal(hc[0], hb2[0])
+    assert same(state(c), state(b))
+    bt = mgfvm.Batch([(pin, ppat), (pin, ppat)])
+    bt.set_inlet_profile(1, prof)
+    nb, cb = bt.solve()
+    assert (nb[1], cb[1]) == (ha[1], ha[2]) and (nb[0], cb[0]) == (hc[1], hc[2])
+    for ig in range(prob.ngrid):
+        for n in STATE:
+            assert np.array_equal(bt.get(1, ig, n).view(np.uint64), a.get(ig, n).view(np.uint64))
+            assert np.array_equal(bt.get(0, ig, n).view(np.uint64), c.get(ig, n).view(np.uint64))
+    for s in (a, b, c, bt):
+        s.close()
