@@ -316,3 +316,33 @@ def test_face_prolongation_reproduces_linear_corrections(tmp_path):
     want = lin(xf, yf)
     # fine rows 2..nyf-3, faces 3..nxf-5: both coarse rows and both coarse faces interior
     np.testing.assert_allclose(got[2:-2, 3:-4], want[2:-2, 3:-4], rtol=0, atol=1e-14)
+
+
+def test_jacobi_uses_previous_sweep_only(tmp_path):
+    """Two Jacobi sweeps from zero (V6:1231-1296): the first gives su / ap at
+    every point, and the second point's value depends on the first sweep's
+    neighbours only, so the result is the same whatever the visiting order
+    (here: computed in reverse order with numpy); a Gauss-Seidel update in
+    place fails it."""
+    prob = Problem(nfx=8, nfy=8, ngrid=1, ig_obs=1, isolve=1, nswpp=1, obstacles=[])
+    o = _oracle(tmp_path, prob)
+    co, ap, su = _random_system(o, 7)
+    o.set(0, "pp", np.zeros((10, 10)))
+    o.call("jacobi_pp", 0)
+    x1 = o.get(0, "pp").copy()
+    np.testing.assert_array_equal(x1[1:-1, 1:-1], su[1:-1, 1:-1] / (ap[1:-1, 1:-1] + 1e-30))
+    o.close()
+    prob2 = prob.replace(nswpp=2)
+    o = _oracle(tmp_path, prob2, "c2")
+    co, ap, su = _random_system(o, 7)
+    o.set(0, "pp", np.zeros((10, 10)))
+    o.call("jacobi_pp", 0)
+    x2 = o.get(0, "pp")
+    want = np.zeros((10, 10))
+    for j in range(8, 0, -1):
+        for i in range(8, 0, -1):
+            s = ((co["aw"][j, i] * x1[j, i - 1] + co["ae"][j, i] * x1[j, i + 1]) +
+                 co["as"][j, i] * x1[j - 1, i]) + co["an"][j, i] * x1[j + 1, i]
+            want[j, i] = (s + su[j, i]) / (ap[j, i] + 1e-30)
+    np.testing.assert_allclose(x2[1:-1, 1:-1], want[1:-1, 1:-1], rtol=1e-14, atol=0)
+    o.close()
