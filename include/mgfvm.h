@@ -252,7 +252,9 @@ int mgf_set_inlet_profile(mgf_solver *s, int b, const double *inlet_u);
 /* ---- measurement ------------------------------------------------------------ */
 
 /* Kernel ids for mgf_time_kernel */
-#define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation */
+#define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation (FP64
+                                    red-black: timed as relaxations issue them, three
+                                    sweeps per launch) */
 #define MGF_KERNEL_SWEEP_U   1   /* one full smoother sweep of the u equation */
 #define MGF_KERNEL_RELAX_U   2   /* mixed red-black: the first launch of the u
                                     relaxation (assembly + sweeps in one kernel) */

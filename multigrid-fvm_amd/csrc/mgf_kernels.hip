@@ -1106,6 +1106,85 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
     }
 }
 
+/* S red-black sweeps per launch in FP64 (the p' solve's nswpp sweeps in
+ * ceil(nswpp / 3) launches instead of nswpp), temporally blocked as the mixed
+ * path's FP32 sweeps are: a workgroup stages its 64 x RB_TY tile with a halo
+ * of H = 2S points from `src` (points outside the swept range from `base`,
+ * the field) and the stored FP64 equation of its points, and colour step m
+ * updates the points at distance >= m from the region's edge, with
+ * k_rb_sweep's expression; after 2S steps the tile holds what S single
+ * sweeps give, bit for bit, and is written to `dst` (ping-pong, so every
+ * workgroup's halo is the launch's old values). */
+#define RBX_NT 512
+template <int S>
+__global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
+{
+    LEVEL(Ls);
+    constexpr int H = 2 * S, RW = BX + 2 * H, RH = RB_TY + 2 * H, NR = RW * RH;
+    constexpr int NPT = (NR + RBX_NT - 1) / RBX_NT;
+    __shared__ real_t t[NR];
+    int nx, ny;
+    const gdbl *base = field_of(L, field, nx, ny);
+    const gdbl *src = parity ? L.tmp : base;
+    gdbl *dst = parity ? (gdbl *)base : L.tmp;
+    const int npx = L.npx, npy = L.npy;
+    const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * RB_TY - H;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    Pt own[NPT];
+    /* per slot: distance to the region's edge (bits 0-7), colour (bit 8),
+     * inside the swept range (bit 9) */
+    unsigned meta[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int p = tid + k * RBX_NT;
+        unsigned m = 0;
+        own[k] = Pt{0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+        if (p < NR) {
+            const int r = p / RW, c = p - r * RW;
+            const int gi = ri0 + c, gj = rj0 + r;
+            real_t val = 0.0;
+            if (gi >= 0 && gi < npx && gj >= 0 && gj < npy) {
+                const int g = gj * npx + gi;
+                const bool in = gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1;
+                val = in ? src[g] : base[g];
+                if (in) {
+                    own[k] = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
+                    const int d = min(min(c, RW - 1 - c), min(r, RH - 1 - r));
+                    m = (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
+                }
+            }
+            t[p] = val;
+        }
+        meta[k] = m;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int step = 1; step <= 2 * S; ++step) {
+        const unsigned want = (1u << 9) | ((unsigned)((step - 1) & 1) << 8);
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const unsigned m = meta[k];
+            if ((m & 0x300u) == want && (int)(m & 0xffu) >= step) {
+                const int p = tid + k * RBX_NT;
+                const Pt &o = own[k];
+                const real_t sum_nb = ((o.aw * t[p - 1] + o.ae * t[p + 1]) + o.as * t[p - RW]) +
+                                      o.an * t[p + RW];
+                t[p] = (sum_nb + o.su) / o.ap;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const unsigned m = meta[k];
+        if ((m & (1u << 9)) && (int)(m & 0xffu) >= H) {
+            const int p = tid + k * RBX_NT;
+            const int r = p / RW, c = p - r * RW;
+            dst[(rj0 + r) * npx + (ri0 + c)] = t[p];
+        }
+    }
+}
+
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
 {
     int nx, ny;
@@ -1121,6 +1200,29 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
                            nsw);
         return;
     }
+#ifndef MGF_TANGENT
+    if (isolve == 2 && nsw >= 3) {
+        /* three sweeps per launch (the last launch the rest); the launch
+         * count's parity says where the result is */
+        const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY);
+        const dim3 b(BX, RBX_NT / BX);
+        int launches = 0;
+        for (int done = 0; done < nsw; ++launches) {
+            const int n = nsw - done < 3 ? nsw - done : 3;
+            if (n == 3)
+                hipLaunchKernelGGL(k_rb_sweeps<3>, g, b, 0, st, G.d, field, launches & 1);
+            else if (n == 2)
+                hipLaunchKernelGGL(k_rb_sweeps<2>, g, b, 0, st, G.d, field, launches & 1);
+            else
+                hipLaunchKernelGGL(k_rb_sweep, g, dim3(BX, BY), 0, st, G.d, field, launches & 1);
+            done += n;
+        }
+        if (launches & 1)
+            hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0,
+                               st, G.d, field);
+        return;
+    }
+#endif
     for (int it = 0; it < nsw; ++it) {
         if (isolve == 1)
             hipLaunchKernelGGL(k_jacobi, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st, G.d,
