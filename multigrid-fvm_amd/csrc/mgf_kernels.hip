@@ -1106,8 +1106,8 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
     }
 }
 
-/* S red-black sweeps per launch in FP64 (the p' solve's nswpp sweeps in
- * ceil(nswpp / 3) launches instead of nswpp), temporally blocked as the mixed
+/* S red-black sweeps per launch in FP64 (a solve's nsw sweeps in
+ * ceil(nsw / 3) launches instead of nsw), temporally blocked as the mixed
  * path's FP32 sweeps are: a workgroup stages its 64 x RB_TY tile with a halo
  * of H = 2S points from `src` (points outside the swept range from `base`,
  * the field) and the stored FP64 equation of its points, and colour step m
@@ -1116,6 +1116,12 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
  * sweeps give, bit for bit, and is written to `dst` (ping-pong, so every
  * workgroup's halo is the launch's old values). */
 #define RBX_NT 512
+/* relaxations of at least this many sweeps take the blocked launches (two
+ * sweeps as one launch and a copy-back already beat two single sweeps: the
+ * equation is read once) */
+#ifndef MGF_RBX_MIN
+#define MGF_RBX_MIN 2
+#endif
 template <int S>
 __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
 {
@@ -1201,7 +1207,7 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
         return;
     }
 #ifndef MGF_TANGENT
-    if (isolve == 2 && nsw >= 3) {
+    if (isolve == 2 && nsw >= MGF_RBX_MIN) {
         /* three sweeps per launch (the last launch the rest); the launch
          * count's parity says where the result is */
         const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY);
