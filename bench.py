@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # 512^2 members): rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
 # `bench.py --roofline-only --reps 10`, each scaled by what the counter reads
 # for a 1 GiB device copy (profiles/traffic_relax_uvs_512x84.md)
-TRAFFIC = 2241.1e6
+TRAFFIC = 2241.3e6
 WORKLOAD = ("256 independent channel problems solved as one batch: n in {128,256,512} "
             "(coarsest level 16x16), staggered obstacle array (spacing 4 on a 16x16 base, "
             "each obstacle moved by a seeded quarter/half/full cell), Re in "
