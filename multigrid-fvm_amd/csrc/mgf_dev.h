@@ -159,6 +159,10 @@ void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
  * Gauss-Seidel (the labelled variant; rows <= LEX_MAX_ROWS) */
 #define LEX_MAX_ROWS 1024
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
+/* FP64 red-black of u (0), v (1) or sc (3) with nsw <= 3: assembly and
+ * sweeps in one launch (replaces mgf_launch_asm_* + mgf_launch_smooth);
+ * returns 0, launching nothing, when not applicable */
+int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1);
  * the merged relaxation's u + du, v + dv in uc, vc, the sc correction in
  * ds32 and the updated p of the copy origins in tmp (2) */

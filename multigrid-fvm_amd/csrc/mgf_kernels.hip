@@ -37,54 +37,83 @@ __global__ void __launch_bounds__(BX * BY) k_fluxes(const DevLevel *Ls)
 /* ---------------------------------------------------------------------- */
 /* coefu + walls + umom, V6:741-767, 815-860: j in [1,npy-2], i in [1,npx-3] */
 /* ---------------------------------------------------------------------- */
-__global__ void __launch_bounds__(BX * BY) k_asm_u(const DevLevel *Ls)
+/* the stored FP64 equation of one point: coefficients, diagonal, source */
+struct Pt {
+    real_t aw, ae, as, an, ap, su;
+};
+
+/* coefu + walls + umom at u face (i, j), V6:741-767, 815-860 */
+__device__ __forceinline__ Pt asm64_u(const DevLevelG &L, int i, int j)
 {
-    LEVEL(Ls);
     const DevPhys &P = L.P;
-    RANGE2D(1, L.npx - 3, 1, L.npy - 2);
     const int ij = j * L.npx + i;
     Coef c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
     real_t su = L.ax * (L.p[ij] - L.p[ij + 1]);
     real_t ap = c.ap / P.relxm;
     su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
     ap = ap + ((L.flags[ij] & MGF_F_SPU) ? 1.0e30 : 0.0);
-    L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
-    L.ap[ij] = ap;
-    L.su[ij] = su;
-    L.apu[ij] = ap;
+    return Pt{c.aw, c.ae, c.as, c.an, ap, su};
 }
 
-/* coefv + side walls + vmom, V6:769-793, 966-1016: j in [1,npy-3], i in [1,npx-2] */
-__global__ void __launch_bounds__(BX * BY) k_asm_v(const DevLevel *Ls)
+/* coefv + side walls + vmom at v face (i, j), V6:769-793, 966-1016 */
+__device__ __forceinline__ Pt asm64_v(const DevLevelG &L, int i, int j)
 {
-    LEVEL(Ls);
     const DevPhys &P = L.P;
-    RANGE2D(1, L.npx - 2, 1, L.npy - 3);
     const int ij = j * L.npx + i;
     Coef c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
     real_t su = L.ay * (L.p[ij] - L.p[ij + L.npx]);
     real_t ap = c.ap / P.relxm;
     su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
     ap = ap + ((L.flags[ij] & MGF_F_SPV) ? 1.0e30 : 0.0);
-    L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
-    L.ap[ij] = ap;
-    L.su[ij] = su;
-    L.apv[ij] = ap;
+    return Pt{c.aw, c.ae, c.as, c.an, ap, su};
 }
 
-/* coefsc + obstacle adds + scalar, V6:795-812, 861-964: [1,npy-2]x[1,npx-2] */
-__global__ void __launch_bounds__(BX * BY) k_asm_sc(const DevLevel *Ls)
+/* coefsc + obstacle adds + scalar at cell (i, j), V6:795-812, 861-964 */
+__device__ __forceinline__ Pt asm64_sc(const DevLevelG &L, int i, int j)
 {
-    LEVEL(Ls);
     const DevPhys &P = L.P;
-    RANGE2D(1, L.npx - 2, 1, L.npy - 2);
     const int ij = j * L.npx + i;
     const uint8_t fl = L.flags[ij];
     Coef c = coefsc_at(L, FluxArr{L.cx, L.cy}, i, j, fl);
     const double spsc = (fl & MGF_F_SPSC) ? 1.0e30 : 0.0;
-    L.aw[ij] = c.aw; L.ae[ij] = c.ae; L.as[ij] = c.as; L.an[ij] = c.an;
-    L.su[ij] = spsc * P.sc_obs + L.ressc[ij];
-    L.ap[ij] = c.ap + spsc;
+    return Pt{c.aw, c.ae, c.as, c.an, c.ap + spsc, spsc * P.sc_obs + L.ressc[ij]};
+}
+
+__device__ __forceinline__ void store_eq(const DevLevelG &L, int ij, const Pt &q)
+{
+    L.aw[ij] = q.aw; L.ae[ij] = q.ae; L.as[ij] = q.as; L.an[ij] = q.an;
+    L.ap[ij] = q.ap;
+    L.su[ij] = q.su;
+}
+
+/* j in [1,npy-2], i in [1,npx-3] */
+__global__ void __launch_bounds__(BX * BY) k_asm_u(const DevLevel *Ls)
+{
+    LEVEL(Ls);
+    RANGE2D(1, L.npx - 3, 1, L.npy - 2);
+    const int ij = j * L.npx + i;
+    const Pt q = asm64_u(L, i, j);
+    store_eq(L, ij, q);
+    L.apu[ij] = q.ap;
+}
+
+/* j in [1,npy-3], i in [1,npx-2] */
+__global__ void __launch_bounds__(BX * BY) k_asm_v(const DevLevel *Ls)
+{
+    LEVEL(Ls);
+    RANGE2D(1, L.npx - 2, 1, L.npy - 3);
+    const int ij = j * L.npx + i;
+    const Pt q = asm64_v(L, i, j);
+    store_eq(L, ij, q);
+    L.apv[ij] = q.ap;
+}
+
+/* [1,npy-2]x[1,npx-2] */
+__global__ void __launch_bounds__(BX * BY) k_asm_sc(const DevLevel *Ls)
+{
+    LEVEL(Ls);
+    RANGE2D(1, L.npx - 2, 1, L.npy - 2);
+    store_eq(L, j * L.npx + i, asm64_sc(L, i, j));
 }
 
 /* ---------------------------------------------------------------------- */
@@ -196,10 +225,6 @@ __global__ void __launch_bounds__(BX * BY) k_copy_range(const DevLevel *Ls, int 
 #define RB_TY 16
 #define RB_Q (RB_TY / BY)
 #define RB_RING (2 * (BX + 2) + 2 * RB_TY)
-
-struct Pt {
-    real_t aw, ae, as, an, ap, su;
-};
 
 __device__ __forceinline__ void rb_ring_pos(int k, int &r, int &c)
 {
@@ -1122,7 +1147,7 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
 #ifndef MGF_RBX_MIN
 #define MGF_RBX_MIN 2
 #endif
-template <int S>
+template <int S, int ASMF = -1>
 __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
 {
     LEVEL(Ls);
@@ -1154,7 +1179,14 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
                 const bool in = gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1;
                 val = in ? src[g] : base[g];
                 if (in) {
-                    own[k] = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
+                    if constexpr (ASMF == 0)
+                        own[k] = asm64_u(L, gi, gj);
+                    else if constexpr (ASMF == 1)
+                        own[k] = asm64_v(L, gi, gj);
+                    else if constexpr (ASMF == 3)
+                        own[k] = asm64_sc(L, gi, gj);
+                    else
+                        own[k] = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
                     const int d = min(min(c, RW - 1 - c), min(r, RH - 1 - r));
                     m = (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
                 }
@@ -1186,9 +1218,53 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
         if ((m & (1u << 9)) && (int)(m & 0xffu) >= H) {
             const int p = tid + k * RBX_NT;
             const int r = p / RW, c = p - r * RW;
-            dst[(rj0 + r) * npx + (ri0 + c)] = t[p];
+            const int g = (rj0 + r) * npx + (ri0 + c);
+            dst[g] = t[p];
+            if constexpr (ASMF == 0) L.apu[g] = own[k].ap;
+            if constexpr (ASMF == 1) L.apv[g] = own[k].ap;
         }
     }
+}
+
+/* A whole FP64 red-black solve of u, v or sc with at most three sweeps in
+ * one launch that also assembles the equation (k_asm_u / k_asm_v /
+ * k_asm_sc's expressions, at every point of the region: the halo's points
+ * are assembled again by each workgroup that needs them, from inputs the
+ * launch does not write), then the copy-back; the equation never crosses
+ * HBM.  0 when not applicable (another smoother or more sweeps). */
+template <int F>
+static void launch_relax64_f(const DevGroup &G, int nsw, const dim3 &g, hipStream_t st)
+{
+    const dim3 b(BX, RBX_NT / BX);
+    if (nsw == 1)
+        hipLaunchKernelGGL((k_rb_sweeps<1, F>), g, b, 0, st, G.d, F, 0);
+    else if (nsw == 2)
+        hipLaunchKernelGGL((k_rb_sweeps<2, F>), g, b, 0, st, G.d, F, 0);
+    else
+        hipLaunchKernelGGL((k_rb_sweeps<3, F>), g, b, 0, st, G.d, F, 0);
+}
+
+int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
+{
+#ifdef MGF_TANGENT
+    (void)G; (void)field; (void)isolve; (void)nsw; (void)st;
+    return 0;
+#else
+    if (isolve != 2 || nsw < 1 || nsw > 3 || field == 2) return 0;
+    const int nx = field == 0 ? G.npx - 2 : G.npx - 1;
+    const int ny = field == 1 ? G.npy - 2 : G.npy - 1;
+    if (nx - 1 < 1 || ny - 1 < 1) return 0;
+    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY);
+    if (field == 0)
+        launch_relax64_f<0>(G, nsw, g, st);
+    else if (field == 1)
+        launch_relax64_f<1>(G, nsw, g, st);
+    else
+        launch_relax64_f<3>(G, nsw, g, st);
+    hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                       G.d, field);
+    return 1;
+#endif
 }
 
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)

@@ -727,23 +727,24 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
     if (s->opt.precision == MGF_PREC_MIXED) return enqueue_relax_mixed(s, G, leg, err, R);
     hipStream_t st = s->est;
     mgf_launch_fluxes(G, st);
-    mgf_launch_asm_u(G, st);
-    mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
-    mgf_launch_asm_v(G, st);
-    mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
+    if (!mgf_launch_relax64(G, 0, s->smoother, s->nswpm, st)) {
+        mgf_launch_asm_u(G, st);
+        mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
+    }
+    if (!mgf_launch_relax64(G, 1, s->smoother, s->nswpm, st)) {
+        mgf_launch_asm_v(G, st);
+        mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
+    }
     mgf_launch_asm_p(G, err, st);
     mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
     mgf_launch_update(G, 0, st);
     mgf_launch_pbound(G, st);
-    if (leg == 0) {
-        mgf_launch_outflow(G, st);
+    if (leg == 0) mgf_launch_outflow(G, st);
+    if (!mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st)) {
         mgf_launch_asm_sc(G, st);
         mgf_launch_smooth(G, 3, s->smoother, s->nswpm, st);
-    } else {
-        mgf_launch_asm_sc(G, st);
-        mgf_launch_smooth(G, 3, s->smoother, s->nswpm, st);
-        mgf_launch_outflow(G, st);
     }
+    if (leg == 1) mgf_launch_outflow(G, st);
     return 0;
 }
 
