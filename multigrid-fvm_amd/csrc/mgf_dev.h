@@ -163,6 +163,9 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
  * sweeps in one launch (replaces mgf_launch_asm_* + mgf_launch_smooth);
  * returns 0, launching nothing, when not applicable */
 int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
+/* the same for p' (any nsw; err as mgf_launch_asm_p; replaces
+ * mgf_launch_asm_p + mgf_launch_smooth) */
+int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, hipStream_t st);
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1);
  * the merged relaxation's u + du, v + dv in uc, vc, the sc correction in
  * ds32 and the updated p of the copy origins in tmp (2) */

@@ -79,6 +79,29 @@ __device__ __forceinline__ Pt asm64_sc(const DevLevelG &L, int i, int j)
     return Pt{c.aw, c.ae, c.as, c.an, c.ap + spsc, spsc * P.sc_obs + L.ressc[ij]};
 }
 
+/* masserror + coefp (+ p' Dirichlet rows) at cell (i, j), V6:1018-1125;
+ * mag = |div| */
+__device__ __forceinline__ Pt asm64_p(const DevLevelG &L, int i, int j, double &mag)
+{
+    const DevPhys &P = L.P;
+    const int nx = L.npx;
+    const int ij = j * nx + i, ijs = ij - nx;
+    real_t div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
+    real_t su = -div;
+    mag = val(fabs(div));
+    real_t aw = L.axax / L.apu[ij - 1];
+    real_t ae = L.axax / L.apu[ij];
+    real_t as = L.ayay / L.apv[ijs];
+    real_t an = L.ayay / L.apv[ij];
+    real_t ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
+    const double diag = 1.0e30;
+    if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
+    if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
+    if (P.ibc[2] == 2 && j == 1) { ap = ap + diag; su = 0.0; }
+    if (P.ibc[3] == 2 && j == L.npy - 2) { ap = ap + diag; su = 0.0; }
+    return Pt{aw, ae, as, an, ap, su};
+}
+
 __device__ __forceinline__ void store_eq(const DevLevelG &L, int ij, const Pt &q)
 {
     L.aw[ij] = q.aw; L.ae[ij] = q.ae; L.as[ij] = q.as; L.an[ij] = q.an;
@@ -130,24 +153,9 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
     const int nx = L.npx;
     double mag = 0.0;
     if (i <= nx - 2 && j <= L.npy - 2) {
-        const int ij = j * nx + i, ijs = ij - nx;
-        real_t div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
-        real_t su = -div;
-        mag = val(fabs(div));
-        real_t aw = L.axax / L.apu[ij - 1];
-        real_t ae = L.axax / L.apu[ij];
-        real_t as = L.ayay / L.apv[ijs];
-        real_t an = L.ayay / L.apv[ij];
-        real_t ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
-        const double diag = 1.0e30;
-        if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
-        if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
-        if (P.ibc[2] == 2 && j == 1) { ap = ap + diag; su = 0.0; }
-        if (P.ibc[3] == 2 && j == L.npy - 2) { ap = ap + diag; su = 0.0; }
-        L.su[ij] = su;
+        const int ij = j * nx + i;
+        store_eq(L, ij, asm64_p(L, i, j, mag));
         L.pp[ij] = 0.0;
-        L.aw[ij] = aw; L.ae[ij] = ae; L.as[ij] = as; L.an[ij] = an;
-        L.ap[ij] = ap;
     }
     double bs = block_sum(mag, lds);
     const int pb = T.y * (int)gridDim.x + T.x;     /* padding rows have none */
@@ -1147,13 +1155,17 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
 #ifndef MGF_RBX_MIN
 #define MGF_RBX_MIN 2
 #endif
-template <int S, int ASMF = -1>
+template <int S, int ASMF = -1, bool STORE = false>
 __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
 {
     LEVEL(Ls);
     constexpr int H = 2 * S, RW = BX + 2 * H, RH = RB_TY + 2 * H, NR = RW * RH;
     constexpr int NPT = (NR + RBX_NT - 1) / RBX_NT;
+    /* p': the tile's |div| per point, then per tile row, for the partial sums */
+    constexpr bool PF = ASMF == 2;
     __shared__ real_t t[NR];
+    __shared__ double mg[PF ? RB_TY * BX : 1];
+    __shared__ double rows[PF ? RB_TY : 1];
     int nx, ny;
     const gdbl *base = field_of(L, field, nx, ny);
     const gdbl *src = parity ? L.tmp : base;
@@ -1161,6 +1173,10 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
     const int npx = L.npx, npy = L.npy;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * RB_TY - H;
     const int tid = threadIdx.y * BX + threadIdx.x;
+    if constexpr (PF) {
+        for (int k = tid; k < RB_TY * BX; k += RBX_NT) mg[k] = 0.0;
+        __syncthreads();
+    }
     Pt own[NPT];
     /* per slot: distance to the region's edge (bits 0-7), colour (bit 8),
      * inside the swept range (bit 9) */
@@ -1177,17 +1193,23 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
             if (gi >= 0 && gi < npx && gj >= 0 && gj < npy) {
                 const int g = gj * npx + gi;
                 const bool in = gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1;
-                val = in ? src[g] : base[g];
+                const int d = min(min(c, RW - 1 - c), min(r, RH - 1 - r));
+                /* p' starts every relaxation from zero (masserror, V6:1116) */
+                val = in ? (PF ? real_t(0.0) : src[g]) : base[g];
                 if (in) {
-                    if constexpr (ASMF == 0)
+                    if constexpr (ASMF == 0) {
                         own[k] = asm64_u(L, gi, gj);
-                    else if constexpr (ASMF == 1)
+                    } else if constexpr (ASMF == 1) {
                         own[k] = asm64_v(L, gi, gj);
-                    else if constexpr (ASMF == 3)
+                    } else if constexpr (ASMF == 2) {
+                        double mag;
+                        own[k] = asm64_p(L, gi, gj, mag);
+                        if (d >= H) mg[(r - H) * BX + (c - H)] = mag;
+                    } else if constexpr (ASMF == 3) {
                         own[k] = asm64_sc(L, gi, gj);
-                    else
+                    } else {
                         own[k] = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
-                    const int d = min(min(c, RW - 1 - c), min(r, RH - 1 - r));
+                    }
                     m = (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
                 }
             }
@@ -1222,6 +1244,27 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
             dst[g] = t[p];
             if constexpr (ASMF == 0) L.apu[g] = own[k].ap;
             if constexpr (ASMF == 1) L.apv[g] = own[k].ap;
+            if constexpr (STORE) store_eq(L, g, own[k]);
+        }
+    }
+    if constexpr (PF) {
+        /* the |div| partial sums of k_asm_p, bit for bit: per 64 x 4 block,
+         * each row summed by one wave with block_sum's shuffle tree, then the
+         * block's four rows in order */
+        const int w = threadIdx.y, lane = threadIdx.x;
+        for (int row = w; row < RB_TY; row += RBX_NT / BX) {
+            double v = mg[row * BX + lane];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) rows[row] = v;
+        }
+        __syncthreads();
+        if (tid < RB_TY / BY) {
+            const int q = tid;
+            const double bs = ((rows[q * BY] + rows[q * BY + 1]) + rows[q * BY + 2]) +
+                              rows[q * BY + 3];
+            const int gxp = (npx - 2 + BX - 1) / BX;
+            const int pb = (T.y * (RB_TY / BY) + q) * gxp + T.x;
+            if (pb < L.nblk) L.partial[pb] = bs;
         }
     }
 }
@@ -1242,6 +1285,47 @@ static void launch_relax64_f(const DevGroup &G, int nsw, const dim3 &g, hipStrea
         hipLaunchKernelGGL((k_rb_sweeps<2, F>), g, b, 0, st, G.d, F, 0);
     else
         hipLaunchKernelGGL((k_rb_sweeps<3, F>), g, b, 0, st, G.d, F, 0);
+}
+
+/* The FP64 p' relaxation (red-black): its first launch assembles (k_asm_p's
+ * expressions, with the |div| partial sums), runs up to three sweeps from
+ * zero and stores the tile's equation when launches on the stored equation
+ * follow; err as mgf_launch_asm_p.  0 when not applicable. */
+int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, hipStream_t st)
+{
+#ifdef MGF_TANGENT
+    (void)G; (void)isolve; (void)nsw; (void)err; (void)st;
+    return 0;
+#else
+    const int nx = G.npx - 1, ny = G.npy - 1;
+    if (isolve != 2 || nsw < 1 || nx - 1 < 1 || ny - 1 < 1) return 0;
+    const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY), b(BX, RBX_NT / BX);
+    const int n0 = nsw < 3 ? nsw : 3;
+    if (n0 == 3) {
+        if (nsw > 3) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true>), g, b, 0, st, G.d, 2, 0);
+        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false>), g, b, 0, st, G.d, 2, 0);
+    } else if (n0 == 2) {
+        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false>), g, b, 0, st, G.d, 2, 0);
+    } else {
+        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false>), g, b, 0, st, G.d, 2, 0);
+    }
+    if (err) mgf_launch_reduce_err(G, err == 2, st);
+    int launches = 1;
+    for (int done = n0; done < nsw; ++launches) {
+        const int n = nsw - done < 3 ? nsw - done : 3;
+        if (n == 3)
+            hipLaunchKernelGGL(k_rb_sweeps<3>, g, b, 0, st, G.d, 2, launches & 1);
+        else if (n == 2)
+            hipLaunchKernelGGL(k_rb_sweeps<2>, g, b, 0, st, G.d, 2, launches & 1);
+        else
+            hipLaunchKernelGGL(k_rb_sweep, g, dim3(BX, BY), 0, st, G.d, 2, launches & 1);
+        done += n;
+    }
+    if (launches & 1)
+        hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                           G.d, 2);
+    return 1;
+#endif
 }
 
 int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)

@@ -735,8 +735,10 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
         mgf_launch_asm_v(G, st);
         mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
     }
-    mgf_launch_asm_p(G, err, st);
-    mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
+    if (!mgf_launch_relax64_p(G, s->smoother, s->nswpp, err, st)) {
+        mgf_launch_asm_p(G, err, st);
+        mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
+    }
     mgf_launch_update(G, 0, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) mgf_launch_outflow(G, st);
