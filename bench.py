@@ -190,6 +190,16 @@ def roofline_only(args):
     try:
         probs = staggered_batch(args.count)
         pairs = [p.write(os.path.join(work, f"p{k}")) for k, p in enumerate(probs)]
+        if args.fp64_only:
+            bf = mgfvm.Batch(pairs, precision=mgfvm.PREC_FP64, device=0)
+            t0 = time.perf_counter()
+            nf, cf = bf.solve()
+            _sync(bf)
+            tf = time.perf_counter() - t0
+            bf.close()
+            print(f"[bench] batched fp64: {sum(cf)} converged in {tf:.2f}s "
+                  f"({sum(cf) / tf:.2f} solves/s)", file=sys.stderr, flush=True)
+            return
         roof = kernel_roofline(probs, pairs, args.reps, cycles=3)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
@@ -219,6 +229,8 @@ def main():
                     help="launch the batched V-cycles eagerly instead of replaying a hipGraph")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant kernel and a 1 GiB copy (for counter profiles)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="solve only the batch in FP64, once (for kernel traces of that path)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
@@ -242,6 +254,16 @@ def main():
     try:
         probs = staggered_batch(args.count)
         pairs = [p.write(os.path.join(work, f"p{k}")) for k, p in enumerate(probs)]
+        if args.fp64_only:
+            bf = mgfvm.Batch(pairs, precision=mgfvm.PREC_FP64, device=0)
+            t0 = time.perf_counter()
+            nf, cf = bf.solve()
+            _sync(bf)
+            tf = time.perf_counter() - t0
+            bf.close()
+            print(f"[bench] batched fp64: {sum(cf)} converged in {tf:.2f}s "
+                  f"({sum(cf) / tf:.2f} solves/s)", file=sys.stderr, flush=True)
+            return
 
         # ---- batched mixed precision (the measured line) ----------------
         bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, device=0,

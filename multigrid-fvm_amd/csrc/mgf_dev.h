@@ -153,7 +153,8 @@ void mgf_launch_asm_u(const DevGroup &G, hipStream_t st);
 void mgf_launch_asm_v(const DevGroup &G, hipStream_t st);
 /* err: reduce the sum |div| (1), and record it as the history entry of
  * level-0 members (2), or neither (0) */
-void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st);
+/* pend: u and v relaxed with pend (their results in uc, vc) */
+void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st, int pend = 0);
 void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
 /* field: 0 u, 1 v, 2 pp, 3 sc; isolve 1 Jacobi, 2 red-black, 3 lexicographic
  * Gauss-Seidel (the labelled variant; rows <= LEX_MAX_ROWS) */
@@ -161,14 +162,20 @@ void mgf_launch_asm_sc(const DevGroup &G, hipStream_t st);
 void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
 /* FP64 red-black of u (0), v (1) or sc (3) with nsw <= 3: assembly and
  * sweeps in one launch (replaces mgf_launch_asm_* + mgf_launch_smooth);
- * returns 0, launching nothing, when not applicable */
-int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st);
+ * returns 0, launching nothing, when not applicable.  pend (u, v): the
+ * result stays in uc / vc until the update (mgf_launch_update 3), which
+ * needs both u and v relaxed so: mgf_relax64_uv_pending says when */
+int mgf_relax64_uv_pending(const DevGroup &G, int isolve, int nsw);
+int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st,
+                       int pend = 0);
 /* the same for p' (any nsw; err as mgf_launch_asm_p; replaces
- * mgf_launch_asm_p + mgf_launch_smooth) */
-int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, hipStream_t st);
+ * mgf_launch_asm_p + mgf_launch_smooth; pend as mgf_launch_asm_p) */
+int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, int pend,
+                         hipStream_t st);
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1);
  * the merged relaxation's u + du, v + dv in uc, vc, the sc correction in
- * ds32 and the updated p of the copy origins in tmp (2) */
+ * ds32 and the updated p of the copy origins in tmp (2); FP64, the relaxed
+ * u and v in uc, vc (3) */
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);

@@ -80,13 +80,19 @@ __device__ __forceinline__ Pt asm64_sc(const DevLevelG &L, int i, int j)
 }
 
 /* masserror + coefp (+ p' Dirichlet rows) at cell (i, j), V6:1018-1125;
- * mag = |div| */
+ * mag = |div|.  PEND: the u and v relaxations left their results over their
+ * swept ranges (u: i <= npx-3, v: j <= npy-3) in uc, vc for the update */
+template <bool PEND = false>
 __device__ __forceinline__ Pt asm64_p(const DevLevelG &L, int i, int j, double &mag)
 {
     const DevPhys &P = L.P;
     const int nx = L.npx;
     const int ij = j * nx + i, ijs = ij - nx;
-    real_t div = ((L.ax * L.u[ij] - L.ax * L.u[ij - 1]) + L.ay * L.v[ij]) - L.ay * L.v[ijs];
+    const auto *ue = (PEND && i <= nx - 3) ? L.uc : L.u;
+    const auto *uw = (PEND && i >= 2) ? L.uc : L.u;
+    const auto *vn = (PEND && j <= L.npy - 3) ? L.vc : L.v;
+    const auto *vs = (PEND && j >= 2) ? L.vc : L.v;
+    real_t div = ((L.ax * ue[ij] - L.ax * uw[ij - 1]) + L.ay * vn[ij]) - L.ay * vs[ijs];
     real_t su = -div;
     mag = val(fabs(div));
     real_t aw = L.axax / L.apu[ij - 1];
@@ -143,6 +149,7 @@ __global__ void __launch_bounds__(BX * BY) k_asm_sc(const DevLevel *Ls)
 /* masserror + coefp (+ p' Dirichlet rows), V6:1018-1125                    */
 /* also the sum of |div| with a deterministic two-level tree               */
 /* ---------------------------------------------------------------------- */
+template <bool PEND>
 __global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
 {
     LEVEL(Ls);
@@ -154,7 +161,7 @@ __global__ void __launch_bounds__(BX * BY) k_asm_p(const DevLevel *Ls)
     double mag = 0.0;
     if (i <= nx - 2 && j <= L.npy - 2) {
         const int ij = j * nx + i;
-        store_eq(L, ij, asm64_p(L, i, j, mag));
+        store_eq(L, ij, asm64_p<PEND>(L, i, j, mag));
         L.pp[ij] = 0.0;
     }
     double bs = block_sum(mag, lds);
@@ -378,7 +385,9 @@ __global__ void k_pstash(const DevLevel *Ls)
  * uc, vc), and its correction in ds32 is
  * applied here (sc += d, as k_apply_mx would; the update does not read sc),
  * and the last p' launch left the updated p of every copy origin in tmp
- * (the value k_pstash would stash) */
+ * (the value k_pstash would stash); 3: FP64, the u and v relaxations left
+ * their results over their swept ranges in uc, vc (the fields hold the old
+ * values there) */
 struct Cell4 {
     real_t u, v, p, sc;
 };
@@ -393,8 +402,9 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     const real_t pp = L.pp[ij];
     const uint8_t fl = L.flags[ij];
     real_t u, v;
-    if (PENDING == 2) {
-        /* the merged relaxation formed u + du, v + dv in uc, vc */
+    if (PENDING >= 2) {
+        /* the merged relaxation formed u + du, v + dv in uc, vc (2); the FP64
+         * u and v relaxations left their results there (3) */
         u = (i <= nx - 3 ? L.uc : L.u)[ij];
         v = (j <= L.npy - 3 ? L.vc : L.v)[ij];
     } else {
@@ -1071,9 +1081,14 @@ void mgf_launch_reduce_err(const DevGroup &G, int record, hipStream_t st)
                        record);
 }
 
-void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st)
+void mgf_launch_asm_p(const DevGroup &G, int err, hipStream_t st, int pend)
 {
-    hipLaunchKernelGGL(k_asm_p, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY), 0, st, G.d);
+    if (pend)
+        hipLaunchKernelGGL(k_asm_p<true>, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY),
+                           0, st, G.d);
+    else
+        hipLaunchKernelGGL(k_asm_p<false>, tile_grid(G.npx - 2, G.npy - 2, G.count), dim3(BX, BY),
+                           0, st, G.d);
     if (err) mgf_launch_reduce_err(G, err == 2, st);
 }
 
@@ -1155,7 +1170,7 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
 #ifndef MGF_RBX_MIN
 #define MGF_RBX_MIN 2
 #endif
-template <int S, int ASMF = -1, bool STORE = false>
+template <int S, int ASMF = -1, bool STORE = false, bool PEND = false>
 __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
 {
     LEVEL(Ls);
@@ -1170,6 +1185,10 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
     const gdbl *base = field_of(L, field, nx, ny);
     const gdbl *src = parity ? L.tmp : base;
     gdbl *dst = parity ? (gdbl *)base : L.tmp;
+    /* PEND u, v: the result stays in uc / vc for the p' assembly and the
+     * update (the field keeps its old values, so no copy-back) */
+    if constexpr (PEND && ASMF == 0) dst = L.uc;
+    if constexpr (PEND && ASMF == 1) dst = L.vc;
     const int npx = L.npx, npy = L.npy;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * RB_TY - H;
     const int tid = threadIdx.y * BX + threadIdx.x;
@@ -1203,7 +1222,7 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
                         own[k] = asm64_v(L, gi, gj);
                     } else if constexpr (ASMF == 2) {
                         double mag;
-                        own[k] = asm64_p(L, gi, gj, mag);
+                        own[k] = asm64_p<PEND>(L, gi, gj, mag);
                         if (d >= H) mg[(r - H) * BX + (c - H)] = mag;
                     } else if constexpr (ASMF == 3) {
                         own[k] = asm64_sc(L, gi, gj);
@@ -1274,41 +1293,54 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
  * k_asm_sc's expressions, at every point of the region: the halo's points
  * are assembled again by each workgroup that needs them, from inputs the
  * launch does not write), then the copy-back; the equation never crosses
- * HBM.  0 when not applicable (another smoother or more sweeps). */
-template <int F>
+ * HBM.  PEND (u, v): no copy-back, the result stays in uc / vc.  0 when not
+ * applicable (another smoother or more sweeps). */
+template <int F, bool PEND>
 static void launch_relax64_f(const DevGroup &G, int nsw, const dim3 &g, hipStream_t st)
 {
     const dim3 b(BX, RBX_NT / BX);
     if (nsw == 1)
-        hipLaunchKernelGGL((k_rb_sweeps<1, F>), g, b, 0, st, G.d, F, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<1, F, false, PEND>), g, b, 0, st, G.d, F, 0);
     else if (nsw == 2)
-        hipLaunchKernelGGL((k_rb_sweeps<2, F>), g, b, 0, st, G.d, F, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<2, F, false, PEND>), g, b, 0, st, G.d, F, 0);
     else
-        hipLaunchKernelGGL((k_rb_sweeps<3, F>), g, b, 0, st, G.d, F, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<3, F, false, PEND>), g, b, 0, st, G.d, F, 0);
 }
 
 /* The FP64 p' relaxation (red-black): its first launch assembles (k_asm_p's
  * expressions, with the |div| partial sums), runs up to three sweeps from
  * zero and stores the tile's equation when launches on the stored equation
  * follow; err as mgf_launch_asm_p.  0 when not applicable. */
-int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, hipStream_t st)
+template <bool PEND>
+static void launch_relax64_p0(const DevGroup &G, int n0, int store, const dim3 &g,
+                              hipStream_t st)
+{
+    const dim3 b(BX, RBX_NT / BX);
+    if (n0 == 3) {
+        if (store) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true, PEND>), g, b, 0, st, G.d, 2, 0);
+        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false, PEND>), g, b, 0, st, G.d, 2, 0);
+    } else if (n0 == 2) {
+        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false, PEND>), g, b, 0, st, G.d, 2, 0);
+    } else {
+        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false, PEND>), g, b, 0, st, G.d, 2, 0);
+    }
+}
+
+int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, int pend,
+                         hipStream_t st)
 {
 #ifdef MGF_TANGENT
-    (void)G; (void)isolve; (void)nsw; (void)err; (void)st;
+    (void)G; (void)isolve; (void)nsw; (void)err; (void)pend; (void)st;
     return 0;
 #else
     const int nx = G.npx - 1, ny = G.npy - 1;
     if (isolve != 2 || nsw < 1 || nx - 1 < 1 || ny - 1 < 1) return 0;
     const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY), b(BX, RBX_NT / BX);
     const int n0 = nsw < 3 ? nsw : 3;
-    if (n0 == 3) {
-        if (nsw > 3) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true>), g, b, 0, st, G.d, 2, 0);
-        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false>), g, b, 0, st, G.d, 2, 0);
-    } else if (n0 == 2) {
-        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false>), g, b, 0, st, G.d, 2, 0);
-    } else {
-        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false>), g, b, 0, st, G.d, 2, 0);
-    }
+    if (pend)
+        launch_relax64_p0<true>(G, n0, nsw > 3, g, st);
+    else
+        launch_relax64_p0<false>(G, n0, nsw > 3, g, st);
     if (err) mgf_launch_reduce_err(G, err == 2, st);
     int launches = 1;
     for (int done = n0; done < nsw; ++launches) {
@@ -1328,10 +1360,22 @@ int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, hipStr
 #endif
 }
 
-int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st)
+int mgf_relax64_uv_pending(const DevGroup &G, int isolve, int nsw)
 {
 #ifdef MGF_TANGENT
-    (void)G; (void)field; (void)isolve; (void)nsw; (void)st;
+    (void)G; (void)isolve; (void)nsw;
+    return 0;
+#else
+    /* both u (npx-3 x npy-2 points) and v (npx-2 x npy-3) take one launch */
+    return isolve == 2 && nsw >= 1 && nsw <= 3 && G.npx - 3 >= 1 && G.npy - 3 >= 1;
+#endif
+}
+
+int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st,
+                       int pend)
+{
+#ifdef MGF_TANGENT
+    (void)G; (void)field; (void)isolve; (void)nsw; (void)st; (void)pend;
     return 0;
 #else
     if (isolve != 2 || nsw < 1 || nsw > 3 || field == 2) return 0;
@@ -1339,14 +1383,20 @@ int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStr
     const int ny = field == 1 ? G.npy - 2 : G.npy - 1;
     if (nx - 1 < 1 || ny - 1 < 1) return 0;
     const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY);
-    if (field == 0)
-        launch_relax64_f<0>(G, nsw, g, st);
+    pend = pend && field != 3;
+    if (field == 0 && pend)
+        launch_relax64_f<0, true>(G, nsw, g, st);
+    else if (field == 0)
+        launch_relax64_f<0, false>(G, nsw, g, st);
+    else if (field == 1 && pend)
+        launch_relax64_f<1, true>(G, nsw, g, st);
     else if (field == 1)
-        launch_relax64_f<1>(G, nsw, g, st);
+        launch_relax64_f<1, false>(G, nsw, g, st);
     else
-        launch_relax64_f<3>(G, nsw, g, st);
-    hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
-                       G.d, field);
+        launch_relax64_f<3, false>(G, nsw, g, st);
+    if (!pend)
+        hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
+                           G.d, field);
     return 1;
 #endif
 }
@@ -1409,7 +1459,9 @@ void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st)
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
     const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
-    if (pending == 2)
+    if (pending == 3)
+        hipLaunchKernelGGL(k_update<3>, g, dim3(BX, BY), 0, st, G.d);
+    else if (pending == 2)
         hipLaunchKernelGGL(k_update<2>, g, dim3(BX, BY), 0, st, G.d);
     else if (pending)
         hipLaunchKernelGGL(k_update<1>, g, dim3(BX, BY), 0, st, G.d);
