@@ -93,8 +93,9 @@ struct DevPhys {
     GQ float *aw32, *ae32, *as32, *an32, *iap32, *r32, *da32, *db32, *du32, *dv32;    \
     GQ float *ds32;                                                                   \
     /* the merged relaxation's u and v with their corrections applied (FP64, over    \
-     * the swept ranges), read by its p' assembly and update instead of u + du */     \
-    GQ real_t *uc, *vc;                                                               \
+     * the swept ranges), read by its p' assembly and update instead of u + du;      \
+     * FP64: the relaxed u, v (and sc in scn) kept there until the update */          \
+    GQ real_t *uc, *vc, *scn;                                                         \
     /* obstacle data */                                                               \
     GQ const uint8_t *flags;                                                          \
     GQ const uint64_t *sccode;                                                        \
@@ -166,6 +167,10 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
  * result stays in uc / vc until the update (mgf_launch_update 3), which
  * needs both u and v relaxed so: mgf_relax64_uv_pending says when */
 int mgf_relax64_uv_pending(const DevGroup &G, int isolve, int nsw);
+/* pend 2 (no outflow boundary, so nothing writes sc's ghosts between the
+ * update and the sc solve): the fluxes are formed from u, v inside each
+ * assembly (no k_fluxes), sc is relaxed before p' with its result in scn,
+ * and the update (4) applies u, v and sc */
 int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStream_t st,
                        int pend = 0);
 /* the same for p' (any nsw; err as mgf_launch_asm_p; replaces
@@ -175,7 +180,7 @@ int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, int pe
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1);
  * the merged relaxation's u + du, v + dv in uc, vc, the sc correction in
  * ds32 and the updated p of the copy origins in tmp (2); FP64, the relaxed
- * u and v in uc, vc (3) */
+ * u and v in uc, vc (3), and sc in scn (4) */
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);

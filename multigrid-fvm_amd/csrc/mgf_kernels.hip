@@ -42,12 +42,25 @@ struct Pt {
     real_t aw, ae, as, an, ap, su;
 };
 
+/* face fluxes of the relaxation: the k_fluxes arrays, or (INL) formed from
+ * u, v with the same expression while u, v still hold the values the
+ * relaxation started from */
+template <bool INL>
+__device__ __forceinline__ auto relax_flux(const DevLevelG &L)
+{
+    if constexpr (INL)
+        return FluxInl{L.u, L.v, L.ax, L.ay};
+    else
+        return FluxArr{L.cx, L.cy};
+}
+
 /* coefu + walls + umom at u face (i, j), V6:741-767, 815-860 */
+template <bool INL = false>
 __device__ __forceinline__ Pt asm64_u(const DevLevelG &L, int i, int j)
 {
     const DevPhys &P = L.P;
     const int ij = j * L.npx + i;
-    Coef c = coefu_at(L, FluxArr{L.cx, L.cy}, i, j);
+    Coef c = coefu_at(L, relax_flux<INL>(L), i, j);
     real_t su = L.ax * (L.p[ij] - L.p[ij + 1]);
     real_t ap = c.ap / P.relxm;
     su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
@@ -56,11 +69,12 @@ __device__ __forceinline__ Pt asm64_u(const DevLevelG &L, int i, int j)
 }
 
 /* coefv + side walls + vmom at v face (i, j), V6:769-793, 966-1016 */
+template <bool INL = false>
 __device__ __forceinline__ Pt asm64_v(const DevLevelG &L, int i, int j)
 {
     const DevPhys &P = L.P;
     const int ij = j * L.npx + i;
-    Coef c = coefv_at(L, FluxArr{L.cx, L.cy}, i, j);
+    Coef c = coefv_at(L, relax_flux<INL>(L), i, j);
     real_t su = L.ay * (L.p[ij] - L.p[ij + L.npx]);
     real_t ap = c.ap / P.relxm;
     su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
@@ -69,12 +83,13 @@ __device__ __forceinline__ Pt asm64_v(const DevLevelG &L, int i, int j)
 }
 
 /* coefsc + obstacle adds + scalar at cell (i, j), V6:795-812, 861-964 */
+template <bool INL = false>
 __device__ __forceinline__ Pt asm64_sc(const DevLevelG &L, int i, int j)
 {
     const DevPhys &P = L.P;
     const int ij = j * L.npx + i;
     const uint8_t fl = L.flags[ij];
-    Coef c = coefsc_at(L, FluxArr{L.cx, L.cy}, i, j, fl);
+    Coef c = coefsc_at(L, relax_flux<INL>(L), i, j, fl);
     const double spsc = (fl & MGF_F_SPSC) ? 1.0e30 : 0.0;
     return Pt{c.aw, c.ae, c.as, c.an, c.ap + spsc, spsc * P.sc_obs + L.ressc[ij]};
 }
@@ -387,7 +402,7 @@ __global__ void k_pstash(const DevLevel *Ls)
  * and the last p' launch left the updated p of every copy origin in tmp
  * (the value k_pstash would stash); 3: FP64, the u and v relaxations left
  * their results over their swept ranges in uc, vc (the fields hold the old
- * values there) */
+ * values there); 4: as 3, and sc's relaxation left its result in scn */
 struct Cell4 {
     real_t u, v, p, sc;
 };
@@ -419,7 +434,7 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     Cell4 c;
     c.u = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
     c.v = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
-    c.sc = L.sc[ij];
+    c.sc = PENDING == 4 ? L.scn[ij] : L.sc[ij];
     if (PENDING == 2) c.sc = pending_value(c.sc, L.ds32[ij], L.pset && (fl & MGF_F_SPSC), P.sc_obs);
     if (fl & MGF_F_PCOPY) {
         if (PENDING == 2) {
@@ -438,7 +453,7 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     L.u[ij] = c.u;
     L.v[ij] = c.v;
     L.p[ij] = c.p;
-    if (PENDING == 2) L.sc[ij] = c.sc;
+    if (PENDING == 2 || PENDING == 4) L.sc[ij] = c.sc;
     return c;
 }
 
@@ -1170,7 +1185,7 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
 #ifndef MGF_RBX_MIN
 #define MGF_RBX_MIN 2
 #endif
-template <int S, int ASMF = -1, bool STORE = false, bool PEND = false>
+template <int S, int ASMF = -1, bool STORE = false, int PEND = 0>
 __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
 {
     LEVEL(Ls);
@@ -1185,10 +1200,13 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
     const gdbl *base = field_of(L, field, nx, ny);
     const gdbl *src = parity ? L.tmp : base;
     gdbl *dst = parity ? (gdbl *)base : L.tmp;
-    /* PEND u, v: the result stays in uc / vc for the p' assembly and the
-     * update (the field keeps its old values, so no copy-back) */
+    /* PEND u, v (and sc with PEND 2): the result stays in uc / vc / scn for
+     * the p' assembly and the update (the field keeps its old values, so no
+     * copy-back); PEND 2: fluxes formed inline (relax_flux) */
+    constexpr bool INL = PEND == 2;
     if constexpr (PEND && ASMF == 0) dst = L.uc;
     if constexpr (PEND && ASMF == 1) dst = L.vc;
+    if constexpr (PEND == 2 && ASMF == 3) dst = L.scn;
     const int npx = L.npx, npy = L.npy;
     const int ri0 = 1 + T.x * BX - H, rj0 = 1 + T.y * RB_TY - H;
     const int tid = threadIdx.y * BX + threadIdx.x;
@@ -1217,15 +1235,15 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
                 val = in ? (PF ? real_t(0.0) : src[g]) : base[g];
                 if (in) {
                     if constexpr (ASMF == 0) {
-                        own[k] = asm64_u(L, gi, gj);
+                        own[k] = asm64_u<INL>(L, gi, gj);
                     } else if constexpr (ASMF == 1) {
-                        own[k] = asm64_v(L, gi, gj);
+                        own[k] = asm64_v<INL>(L, gi, gj);
                     } else if constexpr (ASMF == 2) {
                         double mag;
-                        own[k] = asm64_p<PEND>(L, gi, gj, mag);
+                        own[k] = asm64_p<(PEND != 0)>(L, gi, gj, mag);
                         if (d >= H) mg[(r - H) * BX + (c - H)] = mag;
                     } else if constexpr (ASMF == 3) {
-                        own[k] = asm64_sc(L, gi, gj);
+                        own[k] = asm64_sc<INL>(L, gi, gj);
                     } else {
                         own[k] = Pt{L.aw[g], L.ae[g], L.as[g], L.an[g], L.ap[g], L.su[g]};
                     }
@@ -1295,7 +1313,7 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
  * launch does not write), then the copy-back; the equation never crosses
  * HBM.  PEND (u, v): no copy-back, the result stays in uc / vc.  0 when not
  * applicable (another smoother or more sweeps). */
-template <int F, bool PEND>
+template <int F, int PEND>
 static void launch_relax64_f(const DevGroup &G, int nsw, const dim3 &g, hipStream_t st)
 {
     const dim3 b(BX, RBX_NT / BX);
@@ -1317,12 +1335,12 @@ static void launch_relax64_p0(const DevGroup &G, int n0, int store, const dim3 &
 {
     const dim3 b(BX, RBX_NT / BX);
     if (n0 == 3) {
-        if (store) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true, PEND>), g, b, 0, st, G.d, 2, 0);
-        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false, PEND>), g, b, 0, st, G.d, 2, 0);
+        if (store) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
+        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
     } else if (n0 == 2) {
-        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false, PEND>), g, b, 0, st, G.d, 2, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
     } else {
-        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false, PEND>), g, b, 0, st, G.d, 2, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
     }
 }
 
@@ -1383,17 +1401,23 @@ int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStr
     const int ny = field == 1 ? G.npy - 2 : G.npy - 1;
     if (nx - 1 < 1 || ny - 1 < 1) return 0;
     const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY);
-    pend = pend && field != 3;
-    if (field == 0 && pend)
-        launch_relax64_f<0, true>(G, nsw, g, st);
+    if (field == 3 && pend != 2) pend = 0;
+    if (field == 0 && pend == 2)
+        launch_relax64_f<0, 2>(G, nsw, g, st);
+    else if (field == 0 && pend)
+        launch_relax64_f<0, 1>(G, nsw, g, st);
     else if (field == 0)
-        launch_relax64_f<0, false>(G, nsw, g, st);
+        launch_relax64_f<0, 0>(G, nsw, g, st);
+    else if (field == 1 && pend == 2)
+        launch_relax64_f<1, 2>(G, nsw, g, st);
     else if (field == 1 && pend)
-        launch_relax64_f<1, true>(G, nsw, g, st);
+        launch_relax64_f<1, 1>(G, nsw, g, st);
     else if (field == 1)
-        launch_relax64_f<1, false>(G, nsw, g, st);
+        launch_relax64_f<1, 0>(G, nsw, g, st);
+    else if (pend)
+        launch_relax64_f<3, 2>(G, nsw, g, st);
     else
-        launch_relax64_f<3, false>(G, nsw, g, st);
+        launch_relax64_f<3, 0>(G, nsw, g, st);
     if (!pend)
         hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, field);
@@ -1459,7 +1483,9 @@ void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st)
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
     const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
-    if (pending == 3)
+    if (pending == 4)
+        hipLaunchKernelGGL(k_update<4>, g, dim3(BX, BY), 0, st, G.d);
+    else if (pending == 3)
         hipLaunchKernelGGL(k_update<3>, g, dim3(BX, BY), 0, st, G.d);
     else if (pending == 2)
         hipLaunchKernelGGL(k_update<2>, g, dim3(BX, BY), 0, st, G.d);

@@ -129,8 +129,8 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
-    /* 23 arrays + 4 initial-state copies + uc, vc */
-    size_t t = 29 * align256(n * sizeof(real_t));
+    /* 23 arrays + 4 initial-state copies + uc, vc, scn */
+    size_t t = 30 * align256(n * sizeof(real_t));
     if (mixed) t += 11 * align256(n * sizeof(float));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
@@ -261,6 +261,7 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
         }
         L.uc = (real_t *)take(nb);
         L.vc = (real_t *)take(nb);
+        L.scn = (real_t *)take(nb);
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
         L.pslot = (const int *)take(n * sizeof(int));
@@ -727,10 +728,14 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
 {
     if (s->opt.precision == MGF_PREC_MIXED) return enqueue_relax_mixed(s, G, leg, err, R);
     hipStream_t st = s->est;
-    mgf_launch_fluxes(G, st);
     /* pend: u and v each in one launch whose result stays in uc, vc for the
-     * p' assembly and the update (no copy-back launches) */
-    const int pend = mgf_relax64_uv_pending(G, s->smoother, s->nswpm);
+     * p' assembly and the update (no copy-back launches); 2: also sc, before
+     * p' (its equation reads the fluxes of the u, v the relaxation starts
+     * from, sc and its own terms, none of which u, v or p' change), and the
+     * fluxes formed inside the assemblies */
+    int pend = mgf_relax64_uv_pending(G, s->smoother, s->nswpm);
+    if (pend && !G.any_outflow) pend = 2;
+    if (pend != 2) mgf_launch_fluxes(G, st);
     if (!mgf_launch_relax64(G, 0, s->smoother, s->nswpm, st, pend)) {
         mgf_launch_asm_u(G, st);
         mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
@@ -739,14 +744,15 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
         mgf_launch_asm_v(G, st);
         mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
     }
-    if (!mgf_launch_relax64_p(G, s->smoother, s->nswpp, err, pend, st)) {
-        mgf_launch_asm_p(G, err, st, pend);
+    if (pend == 2) (void)mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st, pend);
+    if (!mgf_launch_relax64_p(G, s->smoother, s->nswpp, err, pend != 0, st)) {
+        mgf_launch_asm_p(G, err, st, pend != 0);
         mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
     }
-    mgf_launch_update(G, pend ? 3 : 0, st);
+    mgf_launch_update(G, pend == 2 ? 4 : pend ? 3 : 0, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) mgf_launch_outflow(G, st);
-    if (!mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st)) {
+    if (pend != 2 && !mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st)) {
         mgf_launch_asm_sc(G, st);
         mgf_launch_smooth(G, 3, s->smoother, s->nswpm, st);
     }
