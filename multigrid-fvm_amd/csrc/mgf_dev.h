@@ -167,6 +167,8 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
  * result stays in uc / vc until the update (mgf_launch_update 3), which
  * needs both u and v relaxed so: mgf_relax64_uv_pending says when */
 int mgf_relax64_uv_pending(const DevGroup &G, int isolve, int nsw);
+/* pend 2's u, v and sc relaxations as one launch (0 when not applicable) */
+int mgf_launch_relax64_uvs(const DevGroup &G, int isolve, int nsw, hipStream_t st);
 /* pend 2 (no outflow boundary, so nothing writes sc's ghosts between the
  * update and the sc solve): the fluxes are formed from u, v inside each
  * assembly (no k_fluxes), sc is relaxed before p' with its result in scn,

@@ -1185,17 +1185,21 @@ __global__ void __launch_bounds__(LEX_MAX_ROWS) k_lexgs(const DevLevel *Ls, int 
 #ifndef MGF_RBX_MIN
 #define MGF_RBX_MIN 2
 #endif
-template <int S, int ASMF = -1, bool STORE = false, int PEND = 0>
-__global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
+template <int S>
+struct RbxShape {
+    static constexpr int H = 2 * S, RW = BX + 2 * H, RH = RB_TY + 2 * H, NR = RW * RH;
+};
+
+/* one workgroup's tile: t (NR), and for p' mg (RB_TY x BX) and rows (RB_TY)
+ * in LDS */
+template <int S, int ASMF, bool STORE, int PEND>
+__device__ __forceinline__ void rb_sweeps_tile(const DevLevelG &L, const TileId &T, int field,
+                                               int parity, real_t *t, double *mg, double *rows)
 {
-    LEVEL(Ls);
     constexpr int H = 2 * S, RW = BX + 2 * H, RH = RB_TY + 2 * H, NR = RW * RH;
     constexpr int NPT = (NR + RBX_NT - 1) / RBX_NT;
     /* p': the tile's |div| per point, then per tile row, for the partial sums */
     constexpr bool PF = ASMF == 2;
-    __shared__ real_t t[NR];
-    __shared__ double mg[PF ? RB_TY * BX : 1];
-    __shared__ double rows[PF ? RB_TY : 1];
     int nx, ny;
     const gdbl *base = field_of(L, field, nx, ny);
     const gdbl *src = parity ? L.tmp : base;
@@ -1306,6 +1310,40 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
     }
 }
 
+template <int S, int ASMF = -1, bool STORE = false, int PEND = 0>
+__global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
+{
+    LEVEL(Ls);
+    constexpr bool PF = ASMF == 2;
+    __shared__ real_t t[RbxShape<S>::NR];
+    __shared__ double mg[PF ? RB_TY * BX : 1];
+    __shared__ double rows[PF ? RB_TY : 1];
+    rb_sweeps_tile<S, ASMF, STORE, PEND>(L, T, field, parity, t, mg, rows);
+}
+
+/* The u, v and sc relaxations of a group in one launch (FP64, the pending
+ * order without an outflow boundary: fluxes formed from u, v, results left
+ * in uc, vc, scn, so none of the three reads what another writes).  The grid
+ * is the sc tile grid with three logical z-slices per member (field = z %
+ * 3); tiles past u's or v's smaller range hold no unknowns and write
+ * nothing. */
+template <int S>
+__global__ void __launch_bounds__(RBX_NT) k_rb_sweeps_uvs(const DevLevel *Ls)
+{
+    const TileId T3 = tile_id();
+    const TileId T{T3.x, T3.y, T3.z / 3};
+    const int f = T3.z - 3 * T.z;
+    const DevLevelG &L = reinterpret_cast<const DevLevelG *>(Ls)[T.z];
+    if (L.ctl->done) return;
+    __shared__ real_t t[RbxShape<S>::NR];
+    if (f == 0)
+        rb_sweeps_tile<S, 0, false, 2>(L, T, 0, 0, t, nullptr, nullptr);
+    else if (f == 1)
+        rb_sweeps_tile<S, 1, false, 2>(L, T, 1, 0, t, nullptr, nullptr);
+    else
+        rb_sweeps_tile<S, 3, false, 2>(L, T, 3, 0, t, nullptr, nullptr);
+}
+
 /* A whole FP64 red-black solve of u, v or sc with at most three sweeps in
  * one launch that also assembles the equation (k_asm_u / k_asm_v /
  * k_asm_sc's expressions, at every point of the region: the halo's points
@@ -1374,6 +1412,27 @@ int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, int pe
     if (launches & 1)
         hipLaunchKernelGGL(k_copy_range, tile_grid(nx - 1, ny - 1, G.count), dim3(BX, BY), 0, st,
                            G.d, 2);
+    return 1;
+#endif
+}
+
+/* u, v and sc with pend 2 in one launch (k_rb_sweeps_uvs); 0 when not
+ * applicable (launching nothing) */
+int mgf_launch_relax64_uvs(const DevGroup &G, int isolve, int nsw, hipStream_t st)
+{
+#ifdef MGF_TANGENT
+    (void)G; (void)isolve; (void)nsw; (void)st;
+    return 0;
+#else
+    if (!mgf_relax64_uv_pending(G, isolve, nsw) || G.any_outflow) return 0;
+    const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, RB_TY);
+    const dim3 g(g0.x, g0.y, 3 * g0.z), b(BX, RBX_NT / BX);
+    if (nsw == 1)
+        hipLaunchKernelGGL(k_rb_sweeps_uvs<1>, g, b, 0, st, G.d);
+    else if (nsw == 2)
+        hipLaunchKernelGGL(k_rb_sweeps_uvs<2>, g, b, 0, st, G.d);
+    else
+        hipLaunchKernelGGL(k_rb_sweeps_uvs<3>, g, b, 0, st, G.d);
     return 1;
 #endif
 }

@@ -736,15 +736,19 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
     int pend = mgf_relax64_uv_pending(G, s->smoother, s->nswpm);
     if (pend && !G.any_outflow) pend = 2;
     if (pend != 2) mgf_launch_fluxes(G, st);
-    if (!mgf_launch_relax64(G, 0, s->smoother, s->nswpm, st, pend)) {
-        mgf_launch_asm_u(G, st);
-        mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
+    if (pend == 2 && mgf_launch_relax64_uvs(G, s->smoother, s->nswpm, st)) {
+        /* u, v and sc in one launch */
+    } else {
+        if (!mgf_launch_relax64(G, 0, s->smoother, s->nswpm, st, pend)) {
+            mgf_launch_asm_u(G, st);
+            mgf_launch_smooth(G, 0, s->smoother, s->nswpm, st);
+        }
+        if (!mgf_launch_relax64(G, 1, s->smoother, s->nswpm, st, pend)) {
+            mgf_launch_asm_v(G, st);
+            mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
+        }
+        if (pend == 2) (void)mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st, pend);
     }
-    if (!mgf_launch_relax64(G, 1, s->smoother, s->nswpm, st, pend)) {
-        mgf_launch_asm_v(G, st);
-        mgf_launch_smooth(G, 1, s->smoother, s->nswpm, st);
-    }
-    if (pend == 2) (void)mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st, pend);
     if (!mgf_launch_relax64_p(G, s->smoother, s->nswpp, err, pend != 0, st)) {
         mgf_launch_asm_p(G, err, st, pend != 0);
         mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
