@@ -1584,7 +1584,9 @@ void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, h
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
     const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
-    if (pending == 2)
+    if (pending == 4)
+        hipLaunchKernelGGL(k_update_restv<4>, g, dim3(BX, BY), 0, st, G.d, R.c);
+    else if (pending == 2)
         hipLaunchKernelGGL(k_update_restv<2>, g, dim3(BX, BY), 0, st, G.d, R.c);
     else if (pending)
         hipLaunchKernelGGL(k_update_restv<1>, g, dim3(BX, BY), 0, st, G.d, R.c);

@@ -753,6 +753,12 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
         mgf_launch_asm_p(G, err, st, pend != 0);
         mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
     }
+    if (pend == 2 && R && !G.any_pbound_x && !G.any_pbound_y) {
+        /* nothing runs between the update and restv (no periodic copies, no
+         * outflow ghosts, sc already relaxed): one launch does both */
+        mgf_launch_update_restv(G, *R, 4, st);
+        return 1;
+    }
     mgf_launch_update(G, pend == 2 ? 4 : pend ? 3 : 0, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) mgf_launch_outflow(G, st);
