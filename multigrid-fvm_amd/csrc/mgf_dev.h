@@ -94,8 +94,9 @@ struct DevPhys {
     GQ float *ds32;                                                                   \
     /* the merged relaxation's u and v with their corrections applied (FP64, over    \
      * the swept ranges), read by its p' assembly and update instead of u + du;      \
-     * FP64: the relaxed u, v (and sc in scn) kept there until the update */          \
-    GQ real_t *uc, *vc, *scn;                                                         \
+     * FP64: the relaxed u, v (and sc in scn) kept there until the update, and      \
+     * in pnew the updated p of the copy origins, written by the last p' launch */    \
+    GQ real_t *uc, *vc, *scn, *pnew;                                                  \
     /* obstacle data */                                                               \
     GQ const uint8_t *flags;                                                          \
     GQ const uint64_t *sccode;                                                        \
@@ -178,11 +179,12 @@ int mgf_launch_relax64(const DevGroup &G, int field, int isolve, int nsw, hipStr
 /* the same for p' (any nsw; err as mgf_launch_asm_p; replaces
  * mgf_launch_asm_p + mgf_launch_smooth; pend as mgf_launch_asm_p) */
 int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, int pend,
-                         hipStream_t st);
+                         hipStream_t st, int *stashed = nullptr);
 /* pending: mixed precision, u and v corrections still in du32 / dv32 (1);
  * the merged relaxation's u + du, v + dv in uc, vc, the sc correction in
  * ds32 and the updated p of the copy origins in tmp (2); FP64, the relaxed
- * u and v in uc, vc (3), and sc in scn (4) */
+ * u and v in uc, vc (3), and sc in scn (4), and the copy origins' updated p
+ * in pnew (5: mgf_launch_relax64_p's *stashed) */
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st);
 void mgf_launch_pbound(const DevGroup &G, hipStream_t st);
 void mgf_launch_outflow(const DevGroup &G, hipStream_t st);

@@ -402,7 +402,8 @@ __global__ void k_pstash(const DevLevel *Ls)
  * and the last p' launch left the updated p of every copy origin in tmp
  * (the value k_pstash would stash); 3: FP64, the u and v relaxations left
  * their results over their swept ranges in uc, vc (the fields hold the old
- * values there); 4: as 3, and sc's relaxation left its result in scn */
+ * values there); 4: as 3, and sc's relaxation left its result in scn; 5: as
+ * 4, and the last p' launch left the copy origins' updated p in pnew */
 struct Cell4 {
     real_t u, v, p, sc;
 };
@@ -434,16 +435,17 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     Cell4 c;
     c.u = u + (L.ax * (pp - L.pp[ij + 1])) / L.apu[ij];
     c.v = v + (L.ay * (pp - L.pp[ij + nx])) / L.apv[ij];
-    c.sc = PENDING == 4 ? L.scn[ij] : L.sc[ij];
+    c.sc = PENDING >= 4 ? L.scn[ij] : L.sc[ij];
     if (PENDING == 2) c.sc = pending_value(c.sc, L.ds32[ij], L.pset && (fl & MGF_F_SPSC), P.sc_obs);
     if (fl & MGF_F_PCOPY) {
-        if (PENDING == 2) {
-            /* the origin's updated p, stashed by the last p' launch; an
-             * origin outside the updated range keeps its p */
+        if (PENDING == 2 || PENDING == 5) {
+            /* the origin's updated p, stashed by the last p' launch (in tmp,
+             * or in pnew for FP64); an origin outside the updated range keeps
+             * its p */
             const int o = L.porig[L.pslot[ij]];
             const int oi = o % nx, oj = o / nx;
             const bool in = (oi >= 1 && oi <= nx - 2 && oj >= 1 && oj <= L.npy - 2);
-            c.p = in ? L.tmp[o] : L.p[o];
+            c.p = in ? (PENDING == 2 ? L.tmp : L.pnew)[o] : L.p[o];
         } else {
             c.p = L.pstash[L.pslot[ij]];
         }
@@ -453,7 +455,7 @@ __device__ __forceinline__ Cell4 update_cell(const DevLevelG &L, int i, int j)
     L.u[ij] = c.u;
     L.v[ij] = c.v;
     L.p[ij] = c.p;
-    if (PENDING == 2 || PENDING == 4) L.sc[ij] = c.sc;
+    if (PENDING == 2 || PENDING >= 4) L.sc[ij] = c.sc;
     return c;
 }
 
@@ -1192,7 +1194,7 @@ struct RbxShape {
 
 /* one workgroup's tile: t (NR), and for p' mg (RB_TY x BX) and rows (RB_TY)
  * in LDS */
-template <int S, int ASMF, bool STORE, int PEND>
+template <int S, int ASMF, bool STORE, int PEND, bool STASH = false>
 __device__ __forceinline__ void rb_sweeps_tile(const DevLevelG &L, const TileId &T, int field,
                                                int parity, real_t *t, double *mg, double *rows)
 {
@@ -1286,6 +1288,9 @@ __device__ __forceinline__ void rb_sweeps_tile(const DevLevelG &L, const TileId 
             if constexpr (ASMF == 0) L.apu[g] = own[k].ap;
             if constexpr (ASMF == 1) L.apv[g] = own[k].ap;
             if constexpr (STORE) store_eq(L, g, own[k]);
+            /* p' (last launch): the copy origins' updated p, as k_pstash */
+            if constexpr (STASH)
+                if (L.flags[g] & MGF_F_PORIG) L.pnew[g] = L.p[g] + L.P.relxp * (t[p] - 0.0);
         }
     }
     if constexpr (PF) {
@@ -1310,7 +1315,7 @@ __device__ __forceinline__ void rb_sweeps_tile(const DevLevelG &L, const TileId 
     }
 }
 
-template <int S, int ASMF = -1, bool STORE = false, int PEND = 0>
+template <int S, int ASMF = -1, bool STORE = false, int PEND = 0, bool STASH = false>
 __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int field, int parity)
 {
     LEVEL(Ls);
@@ -1318,7 +1323,7 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps(const DevLevel *Ls, int fi
     __shared__ real_t t[RbxShape<S>::NR];
     __shared__ double mg[PF ? RB_TY * BX : 1];
     __shared__ double rows[PF ? RB_TY : 1];
-    rb_sweeps_tile<S, ASMF, STORE, PEND>(L, T, field, parity, t, mg, rows);
+    rb_sweeps_tile<S, ASMF, STORE, PEND, STASH>(L, T, field, parity, t, mg, rows);
 }
 
 /* The u, v and sc relaxations of a group in one launch (FP64, the pending
@@ -1337,11 +1342,11 @@ __global__ void __launch_bounds__(RBX_NT) k_rb_sweeps_uvs(const DevLevel *Ls)
     if (L.ctl->done) return;
     __shared__ real_t t[RbxShape<S>::NR];
     if (f == 0)
-        rb_sweeps_tile<S, 0, false, 2>(L, T, 0, 0, t, nullptr, nullptr);
+        rb_sweeps_tile<S, 0, false, 2, false>(L, T, 0, 0, t, nullptr, nullptr);
     else if (f == 1)
-        rb_sweeps_tile<S, 1, false, 2>(L, T, 1, 0, t, nullptr, nullptr);
+        rb_sweeps_tile<S, 1, false, 2, false>(L, T, 1, 0, t, nullptr, nullptr);
     else
-        rb_sweeps_tile<S, 3, false, 2>(L, T, 3, 0, t, nullptr, nullptr);
+        rb_sweeps_tile<S, 3, false, 2, false>(L, T, 3, 0, t, nullptr, nullptr);
 }
 
 /* A whole FP64 red-black solve of u, v or sc with at most three sweeps in
@@ -1367,42 +1372,57 @@ static void launch_relax64_f(const DevGroup &G, int nsw, const dim3 &g, hipStrea
  * expressions, with the |div| partial sums), runs up to three sweeps from
  * zero and stores the tile's equation when launches on the stored equation
  * follow; err as mgf_launch_asm_p.  0 when not applicable. */
-template <bool PEND>
+template <bool PEND, bool STASH>
 static void launch_relax64_p0(const DevGroup &G, int n0, int store, const dim3 &g,
                               hipStream_t st)
 {
+    constexpr int PD = PEND ? 1 : 0;
     const dim3 b(BX, RBX_NT / BX);
     if (n0 == 3) {
-        if (store) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
-        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
+        if (store) hipLaunchKernelGGL((k_rb_sweeps<3, 2, true, PD>), g, b, 0, st, G.d, 2, 0);
+        else hipLaunchKernelGGL((k_rb_sweeps<3, 2, false, PD, STASH>), g, b, 0, st, G.d, 2, 0);
     } else if (n0 == 2) {
-        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<2, 2, false, PD, STASH>), g, b, 0, st, G.d, 2, 0);
     } else {
-        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false, PEND ? 1 : 0>), g, b, 0, st, G.d, 2, 0);
+        hipLaunchKernelGGL((k_rb_sweeps<1, 2, false, PD, STASH>), g, b, 0, st, G.d, 2, 0);
     }
 }
 
 int mgf_launch_relax64_p(const DevGroup &G, int isolve, int nsw, int err, int pend,
-                         hipStream_t st)
+                         hipStream_t st, int *stashed)
 {
 #ifdef MGF_TANGENT
-    (void)G; (void)isolve; (void)nsw; (void)err; (void)pend; (void)st;
+    (void)G; (void)isolve; (void)nsw; (void)err; (void)pend; (void)st; (void)stashed;
     return 0;
 #else
     const int nx = G.npx - 1, ny = G.npy - 1;
     if (isolve != 2 || nsw < 1 || nx - 1 < 1 || ny - 1 < 1) return 0;
     const dim3 g = tile_grid(nx - 1, ny - 1, G.count, RB_TY), b(BX, RBX_NT / BX);
     const int n0 = nsw < 3 ? nsw : 3;
-    if (pend)
-        launch_relax64_p0<true>(G, n0, nsw > 3, g, st);
+    /* stash (in pnew) when asked, copies exist and the last launch is a
+     * blocked one (not the single-sweep kernel of a one-sweep remainder) */
+    const int nlast = nsw <= 3 ? nsw : (nsw - 4) % 3 + 1;
+    const bool stash = stashed && G.max_npcopy > 0 && (nsw <= 3 || nlast >= 2);
+    if (stashed) *stashed = stash;
+    if (pend && stash && nsw <= 3)
+        launch_relax64_p0<true, true>(G, n0, 0, g, st);
+    else if (pend)
+        launch_relax64_p0<true, false>(G, n0, nsw > 3, g, st);
     else
-        launch_relax64_p0<false>(G, n0, nsw > 3, g, st);
+        launch_relax64_p0<false, false>(G, n0, nsw > 3, g, st);
     if (err) mgf_launch_reduce_err(G, err == 2, st);
     int launches = 1;
     for (int done = n0; done < nsw; ++launches) {
         const int n = nsw - done < 3 ? nsw - done : 3;
-        if (n == 3)
+        const bool last = done + n == nsw;
+        if (n == 3 && last && stash)
+            hipLaunchKernelGGL((k_rb_sweeps<3, -1, false, 0, true>), g, b, 0, st, G.d, 2,
+                               launches & 1);
+        else if (n == 3)
             hipLaunchKernelGGL(k_rb_sweeps<3>, g, b, 0, st, G.d, 2, launches & 1);
+        else if (n == 2 && last && stash)
+            hipLaunchKernelGGL((k_rb_sweeps<2, -1, false, 0, true>), g, b, 0, st, G.d, 2,
+                               launches & 1);
         else if (n == 2)
             hipLaunchKernelGGL(k_rb_sweeps<2>, g, b, 0, st, G.d, 2, launches & 1);
         else
@@ -1537,12 +1557,14 @@ void mgf_launch_smooth(const DevGroup &G, int field, int isolve, int nsw, hipStr
 
 void mgf_launch_update(const DevGroup &G, int pending, hipStream_t st)
 {
-    if (G.max_npcopy > 0 && pending != 2)
+    if (G.max_npcopy > 0 && pending != 2 && pending != 5)
         hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
     const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
-    if (pending == 4)
+    if (pending == 5)
+        hipLaunchKernelGGL(k_update<5>, g, dim3(BX, BY), 0, st, G.d);
+    else if (pending == 4)
         hipLaunchKernelGGL(k_update<4>, g, dim3(BX, BY), 0, st, G.d);
     else if (pending == 3)
         hipLaunchKernelGGL(k_update<3>, g, dim3(BX, BY), 0, st, G.d);
@@ -1579,12 +1601,14 @@ void mgf_launch_restrict(const DevPair &R, int restv, hipStream_t st)
 
 void mgf_launch_update_restv(const DevGroup &G, const DevPair &R, int pending, hipStream_t st)
 {
-    if (G.max_npcopy > 0 && pending != 2)
+    if (G.max_npcopy > 0 && pending != 2 && pending != 5)
         hipLaunchKernelGGL(k_pstash, dim3((unsigned)((G.max_npcopy + 255) / 256), 1,
                                           (unsigned)G.count),
                            dim3(256), 0, st, G.d);
     const dim3 g = tile_grid(G.npx - 2, G.npy - 2, G.count);
-    if (pending == 4)
+    if (pending == 5)
+        hipLaunchKernelGGL(k_update_restv<5>, g, dim3(BX, BY), 0, st, G.d, R.c);
+    else if (pending == 4)
         hipLaunchKernelGGL(k_update_restv<4>, g, dim3(BX, BY), 0, st, G.d, R.c);
     else if (pending == 2)
         hipLaunchKernelGGL(k_update_restv<2>, g, dim3(BX, BY), 0, st, G.d, R.c);

@@ -129,8 +129,8 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t level_bytes(const mgf_plan_level *l, int mixed)
 {
     size_t n = (size_t)l->npx * l->npy;
-    /* 23 arrays + 4 initial-state copies + uc, vc, scn */
-    size_t t = 30 * align256(n * sizeof(real_t));
+    /* 23 arrays + 4 initial-state copies + uc, vc, scn, pnew */
+    size_t t = 31 * align256(n * sizeof(real_t));
     if (mixed) t += 11 * align256(n * sizeof(float));
     t += align256((size_t)mgf_partial_blocks(l->npx, l->npy) * sizeof(double));
     t += align256(n);                               /* flags */
@@ -262,6 +262,7 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
         L.uc = (real_t *)take(nb);
         L.vc = (real_t *)take(nb);
         L.scn = (real_t *)take(nb);
+        L.pnew = (real_t *)take(nb);
         L.flags = (const uint8_t *)take(n);
         L.sccode = (const uint64_t *)take(n * sizeof(uint64_t));
         L.pslot = (const int *)take(n * sizeof(int));
@@ -749,17 +750,20 @@ static int enqueue_relax(mgf_solver *s, const DevGroup &G, int leg, int err,
         }
         if (pend == 2) (void)mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st, pend);
     }
-    if (!mgf_launch_relax64_p(G, s->smoother, s->nswpp, err, pend != 0, st)) {
+    int stashed = 0;
+    if (!mgf_launch_relax64_p(G, s->smoother, s->nswpp, err, pend != 0, st,
+                              pend == 2 ? &stashed : nullptr)) {
         mgf_launch_asm_p(G, err, st, pend != 0);
         mgf_launch_smooth(G, 2, s->smoother, s->nswpp, st);
     }
+    const int upd = pend == 2 ? (stashed ? 5 : 4) : pend ? 3 : 0;
     if (pend == 2 && R && !G.any_pbound_x && !G.any_pbound_y) {
         /* nothing runs between the update and restv (no periodic copies, no
          * outflow ghosts, sc already relaxed): one launch does both */
-        mgf_launch_update_restv(G, *R, 4, st);
+        mgf_launch_update_restv(G, *R, upd, st);
         return 1;
     }
-    mgf_launch_update(G, pend == 2 ? 4 : pend ? 3 : 0, st);
+    mgf_launch_update(G, upd, st);
     mgf_launch_pbound(G, st);
     if (leg == 0) mgf_launch_outflow(G, st);
     if (pend != 2 && !mgf_launch_relax64(G, 3, s->smoother, s->nswpm, st)) {
