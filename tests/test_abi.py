@@ -152,6 +152,39 @@ def test_sensitivity_library_exports():
     assert not missing, missing
 
 
+def _dynamic_symbols(path):
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    return sorted(line.split()[-1] for line in out.splitlines() if line.strip())
+
+
+def _header_names(*headers):
+    names = set()
+    for h in headers:
+        txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", h)).read(), flags=re.S)
+        names.update(re.findall(r"\b(mgf_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
+
+
+@pytest.mark.parametrize("lib,headers", [
+    ("libmgfvm.so", ("mgfvm.h",)),
+    ("libmgfvm_tangent.so", ("mgfvm.h", "mgfvm_tangent.h")),
+])
+def test_library_exports_only_its_c_abi(lib, headers):
+    """Each library exports exactly the C entry points its headers declare
+    (no C++ launcher, set-up helper or registration symbol) and binds its own
+    references locally (DF_SYMBOLIC), so the FP64 and the sensitivity
+    library, which share names, can be loaded into one process without one
+    calling into the other."""
+    import subprocess
+    path = os.path.join(ROOT, "multigrid-fvm_amd", lib)
+    assert _dynamic_symbols(path) == _header_names(*headers)
+    dyn = subprocess.run(["readelf", "-d", path], capture_output=True, text=True,
+                         check=True).stdout
+    assert "SYMBOLIC" in dyn
+
+
 def test_options_defaults():
     """The defaults a caller gets: FP64, graphs, red-black, compaction on,
     automatic lanes and launch shapes."""
