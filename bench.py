@@ -328,7 +328,7 @@ def main():
                 _sync(s)
                 dt = time.perf_counter() - t0
                 single[f"{tag}_solves_per_s"] = round(1.0 / dt, 4) if c else 0.0
-                single["v_cycles"] = n
+                single[f"{tag}_v_cycles"] = n
                 if tag == "fp64":
                     us1, b1 = s.time_kernel(mgfvm.KERNEL_SWEEP_PP, 0, 100)
                     single["fp64_sweep_GBps"] = round(b1 / (us1 * 1e-6) / 1e9, 1)
