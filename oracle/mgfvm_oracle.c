@@ -1055,6 +1055,24 @@ int orc_create_from_files(const char *inputdata, const char *pattern, orc_solver
     return rc;
 }
 
+/* the same, with a level-0 inlet u profile (nfy values, rows 1..nfy) set
+ * after init and before rest_geom carries it to the coarse levels */
+int orc_create_from_files_inlet(const char *inputdata, const char *pattern,
+                                const double *inlet_u, orc_solver **out)
+{
+    orc_config c;
+    int rc = orc_parse_files(inputdata, pattern, &c);
+    if (rc) { orc_config_free(&c); *out = NULL; return rc; }
+    if (inlet_u) {
+        c.inlet_u = (double *)malloc(sizeof(double) * (size_t)c.nfy);
+        if (!c.inlet_u) { orc_config_free(&c); *out = NULL; return -1; }
+        memcpy(c.inlet_u, inlet_u, sizeof(double) * (size_t)c.nfy);
+    }
+    rc = orc_create(&c, out);
+    orc_config_free(&c);
+    return rc;
+}
+
 void orc_destroy(orc_solver *s)
 {
     if (!s) return;

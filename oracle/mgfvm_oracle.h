@@ -35,6 +35,8 @@ int orc_parse_files(const char *inputdata, const char *pattern, orc_config *c);
 void orc_config_free(orc_config *c);
 int orc_create(const orc_config *cfg, orc_solver **out);
 int orc_create_from_files(const char *inputdata, const char *pattern, orc_solver **out);
+int orc_create_from_files_inlet(const char *inputdata, const char *pattern,
+                                const double *inlet_u, orc_solver **out);
 void orc_destroy(orc_solver *s);
 void orc_set_verbose(orc_solver *s, int v);
 /* isolve 2 sweeps in lexicographic order instead of red-black (a variant

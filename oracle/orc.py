@@ -35,6 +35,7 @@ def lib():
         vp, cp, ip, dp = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double
         L.orc_last_error.restype = cp
         L.orc_create_from_files.argtypes = [cp, cp, ctypes.POINTER(vp)]
+        L.orc_create_from_files_inlet.argtypes = [cp, cp, ctypes.POINTER(dp), ctypes.POINTER(vp)]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_verbose.argtypes = [vp, ip]
         L.orc_set_gs_lex.argtypes = [vp, ip]
@@ -61,10 +62,18 @@ def _dptr(a: np.ndarray):
 class Oracle:
     """CPU restatement of the reference solver on one InputData/pattern pair."""
 
-    def __init__(self, inputdata: str, pattern: str, gs_lex: bool = False):
+    def __init__(self, inputdata: str, pattern: str, gs_lex: bool = False, inlet_u=None):
+        """inlet_u: optional level-0 inlet u profile (nfy values, rows 1..nfy),
+        set after the initial state and carried to the coarse levels as the
+        boundary restriction carries it (mgfvm_oracle.c init/rest_geom)."""
         L = lib()
         h = ctypes.c_void_p()
-        rc = L.orc_create_from_files(inputdata.encode(), pattern.encode(), ctypes.byref(h))
+        if inlet_u is not None:
+            prof = np.ascontiguousarray(inlet_u, dtype=np.float64)
+            rc = L.orc_create_from_files_inlet(inputdata.encode(), pattern.encode(), _dptr(prof),
+                                               ctypes.byref(h))
+        else:
+            rc = L.orc_create_from_files(inputdata.encode(), pattern.encode(), ctypes.byref(h))
         if rc != 0:
             raise RuntimeError(f"oracle create failed ({rc}): {L.orc_last_error().decode()}")
         self.h = h

@@ -21,6 +21,23 @@ _base = Problem(nfx=32, nfy=16, ngrid=3, ig_obs=3, xl=2.0, yl=0.5, nvcycle=30, n
                 obstacles=[(2, 2, 1, 1), (2, 3, 2, 2), (3, 3, 2, 3), (5, 5, 3, 3),
                            (5, 5, 3, 3), (6, 6, 2, 2)])
 
+# eight two-row bands of a piecewise-constant inlet u profile (mean 1, the
+# east boundary u) as discrete inlets (jf, jl, u, v, sc)
+PROFILE_INLETS = [(1, 2, 0.6, 0.0, 1.0), (3, 4, 0.9, 0.0, 1.0), (5, 6, 1.2, 0.0, 1.0),
+                  (7, 8, 1.3, 0.0, 1.0), (9, 10, 1.1, 0.0, 1.0), (11, 12, 0.8, 0.0, 1.0),
+                  (13, 14, 1.05, 0.0, 1.0), (15, 16, 1.05, 0.0, 1.0)]
+
+
+def profile_of(inlets, nfy):
+    """the level-0 inlet u profile (rows 1..nfy) the discrete inlets set"""
+    prof = [None] * nfy
+    for jf, jl, u, _, _ in inlets:
+        for j in range(jf, jl + 1):
+            prof[j - 1] = u
+    assert None not in prof
+    return prof
+
+
 CASES = {
     "small_jac": _base,
     "small_rb": _base.replace(isolve=2),
@@ -36,6 +53,10 @@ CASES = {
                       amuf=0.005, obstacles=random_mask(16, 0.8, 7)),
     "stag128": Problem(nfx=128, nfy=128, ngrid=3, ig_obs=3, isolve=2, nvcycle=20, nswpp=6,
                        obstacles=staggered_pattern(16, 4, "half", 5)[0]),
+    # discrete inlets on level 0 only (ig_inlt 1) covering every row, with
+    # the west ghost's v and sc: a piecewise-constant inlet u profile in the
+    # reference's own input format (pins the inlet-profile path)
+    "profile_inlets": _base.replace(isolve=2, ig_inlt=1, inlets=PROFILE_INLETS),
     "cfg0": channel_128_inline(),
     "cfg0_rb": channel_128_inline().replace(isolve=2),
 }

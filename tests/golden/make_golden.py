@@ -16,7 +16,7 @@ level, and for the per-routine cases the staged state):
   kat_<case>.json   per-routine digests after each routine (ref_b)
   kat_<case>.npz    the staged state those routines start from
 
-usage: python tests/golden/make_golden.py [--reference /root/reference]
+usage: python tests/golden/make_golden.py [--reference /root/reference] [--only e2e_x ...]
 """
 import argparse
 import hashlib
@@ -109,16 +109,23 @@ def run_kat(name, workdir, nstage=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", nargs="*", default=None,
+                    help="write only these fixtures (e2e_<case> / kat_<case> names)")
     args = ap.parse_args()
+    want = (lambda tag: args.only is None or tag in args.only)
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle", "ref"), f"REF={args.reference}"],
                    check=True)
     with tempfile.TemporaryDirectory() as work:
         for name, prob in E2E.items():
+            if not want(f"e2e_{name}"):
+                continue
             fx = run_e2e(name, prob, work)
             with open(os.path.join(HERE, f"e2e_{name}.json"), "w") as f:
                 json.dump(fx, f, indent=1)
             print(f"e2e {name}: {fx['ncycles']} cycles, converged={fx['converged']}")
         for name in KAT:
+            if not want(f"kat_{name}"):
+                continue
             meta, npz = run_kat(name, work)
             with open(os.path.join(HERE, f"kat_{name}.json"), "w") as f:
                 json.dump(meta, f, indent=1)

@@ -228,5 +228,47 @@ def test_set_inlet_profile_equals_creation_with_it(tmp_path):
         for n in STATE:
             assert np.array_equal(bt.get(1, ig, n).view(np.uint64), a.get(ig, n).view(np.uint64))
             assert np.array_equal(bt.get(0, ig, n).view(np.uint64), c.get(ig, n).view(np.uint64))
+    # the GPU's inlet-profile path against the oracle's, bit for bit (the
+    # oracle's is pinned to the reference by test_oracle_golden)
+    o = orc.Oracle(pin, ppat, inlet_u=prof)
+    ho, no, co = o.solve(prob.nvcycle, prob.convergence)
+    assert (no, co) == (ha[1], ha[2])
+    np.testing.assert_allclose(ha[0], ho, rtol=HIST_RTOL, atol=0)
+    for ig in range(prob.ngrid):
+        for n in STATE:
+            want = o.get(ig, n).view(np.uint64)
+            assert np.array_equal(a.get(ig, n).view(np.uint64), want), (ig, n)
+            assert np.array_equal(bt.get(1, ig, n).view(np.uint64), want), (ig, n)
+    o.close()
     for s in (a, b, c, bt):
         s.close()
+
+
+def test_inlet_profile_bitwise_with_oracle_on_reference_case(tmp_path):
+    """The piecewise-constant profile of the reference-pinned case
+    (discrete inlets on level 0 covering every row, fixture
+    e2e_profile_inlets): the GPU given it as an inlet profile, at creation
+    and through mgf_set_inlet_profile, equals the oracle given the same
+    profile, and the GPU given the discrete inlets, bit for bit on every
+    level."""
+    import mgfvm
+    base = cases.CASES["profile_inlets"]
+    prof = np.array(cases.profile_of(base.inlets, base.nfy))
+    plain = base.replace(ig_inlt=0, inlets=[])
+    pin, ppat = plain.write(str(tmp_path / "plain"))
+    din, dpat = base.write(str(tmp_path / "discrete"))
+    _say("[gpu] inlet profile vs oracle (reference-pinned case)")
+    o = orc.Oracle(pin, ppat, inlet_u=prof)
+    ho, no, co = o.solve(plain.nvcycle, plain.convergence)
+    a = mgfvm.Solver(pin, ppat, inlet_u=prof)
+    b = mgfvm.Solver(pin, ppat)
+    b.set_inlet_profile(prof)
+    d = mgfvm.Solver(din, dpat)
+    for s in (a, b, d):
+        h, n, c = s.solve()
+        assert (n, c) == (no, co)
+        np.testing.assert_allclose(h, ho, rtol=HIST_RTOL, atol=0)
+        bad = _diff_report(s, o, range(o.ngrid))
+        assert not bad, "\n".join(bad)
+        s.close()
+    o.close()

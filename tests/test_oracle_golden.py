@@ -74,3 +74,36 @@ def test_routines_bitwise(path, tmp_path):
                 key = f"L{ig}.{name}"
                 want = rec["changed"].get(key, fx["stage_digests"][key])
                 assert digest(o.get(ig, name)) == want, f"{rec['routine']}@L{rec['level']}: {key}"
+
+
+def test_inlet_profile_equals_discrete_inlets(tmp_path):
+    """The inlet-profile path (a level-0 inlet u profile set after the
+    initial state, carried to the coarse levels by the boundary restriction)
+    is pinned to the reference: a piecewise-constant profile equals the same
+    rows given to the reference as discrete inlets on level 0 (ig_inlt 1,
+    Version-6-26-04.c:645-658) with the west ghost's v and sc, whose coarse
+    inlet values the reference's rest_geom then restricts (V6:1440-1446).
+    The fixture e2e_profile_inlets was written by the reference program."""
+    import cases
+    with open(os.path.join(GOLD, "e2e_profile_inlets.json")) as f:
+        fx = json.load(f)
+    base = cases.CASES["profile_inlets"]
+    assert fx["inputdata"] == base.inputdata_text()
+    prof = cases.profile_of(base.inlets, base.nfy)
+    plain = base.replace(ig_inlt=0, inlets=[])
+    pin, ppat = plain.write(str(tmp_path / "plain"))
+
+    def run(inlet_u):
+        o = orc.Oracle(pin, ppat, inlet_u=inlet_u)
+        hist, nc, conv = o.solve(plain.nvcycle, plain.convergence)
+        dg = [{n: digest(o.get(ig, n)) for n in orc.DUMP_NAMES} for ig in range(o.ngrid)]
+        o.close()
+        return [float(h).hex() for h in hist], nc, conv, dg
+
+    hist, nc, conv, dg = run(prof)
+    assert (nc, conv) == (fx["ncycles"], fx["converged"])
+    assert hist == fx["history_hex"]
+    assert dg == fx["digests"]
+    # without the profile the solve differs (the check is not vacuous)
+    hist0, _, _, dg0 = run(None)
+    assert hist0 != fx["history_hex"] and dg0 != fx["digests"]
