@@ -4,7 +4,11 @@
  * the working directory (or the paths given), echoes the inputs as
  * readInput does (V6:293-397), solves on the GPU, prints one
  * "icycle = %d error = %e" line per V-cycle (the line GUI.py:527 parses),
- * the elapsed time, and writes vtk_acc.vtk and convergence.csv.
+ * the elapsed time, and writes vtk_acc.vtk and convergence.csv: stdout and
+ * both files are the reference program's, byte for byte, except the timing
+ * line's value (and the OBS block of a problem without obstacles, which the
+ * reference leaves without data).  The converged flag and cycle count go to
+ * stderr.
  *
  * usage: mgfvm_run [InputData [pattern.txt]] [--no-graph] [--full-history]
  * Exit status: 0 solved (converged or not), 1 input error, 2 device error,
@@ -93,7 +97,8 @@ int main(int argc, char **argv)
     for (int k = 0; k < ncyc; ++k) printf("icycle = %d error = %e\n", k, hist[k]);
     double secs = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     printf("simple solution took %f seconds to execute \n", secs);
-    printf("converged = %d after %d V-cycles\n", conv, ncyc);
+    /* stdout carries only what the reference prints; the status goes to stderr */
+    fprintf(stderr, "converged = %d after %d V-cycles\n", conv, ncyc);
 
     int npx, npy;
     mgf_level_dims(s, 0, &npx, &npy);

@@ -47,3 +47,35 @@ def test_batch_driver_matches_single_runs(tmp_path):
                 (case / "vtk_acc.vtk").read_bytes(), (re_, k)
             assert (sol / "convergence_files" / f"convergence_{k}.csv").read_bytes() == \
                 (case / "convergence.csv").read_bytes(), (re_, k)
+
+
+def _out_fixtures():
+    import glob
+    return sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "out_*.json")))
+
+
+@pytest.mark.parametrize("path", _out_fixtures(),
+                         ids=[os.path.basename(p)[4:-5] for p in _out_fixtures()])
+def test_driver_outputs_equal_reference_program(tmp_path, path):
+    """mgfvm_run (the drop-in for the reference program) writes the same
+    vtk_acc.vtk and convergence.csv, byte for byte, and the same stdout
+    (input echo, one `icycle = %d error = %e` line per cycle, averages,
+    record count) as the reference program on the same inputs; only the
+    timing line's value differs.  Fixtures written by
+    tests/golden/make_output_golden.py from the unmodified reference."""
+    import hashlib
+    import json
+    with open(path) as f:
+        fx = json.load(f)
+    print(f"[gpu] mgfvm_run outputs vs reference: {fx['case']}", flush=True)
+    (tmp_path / "InputData").write_text(fx["inputdata"])
+    (tmp_path / "pattern.txt").write_text(fx["pattern"])
+    r = subprocess.run([os.path.join(PKG, "mgfvm_run")], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_output_golden import mask_timing
+    assert mask_timing(r.stdout) == fx["stdout"]
+    for name, key in (("vtk_acc.vtk", "vtk_sha256"), ("convergence.csv", "csv_sha256")):
+        got = hashlib.sha256((tmp_path / name).read_bytes()).hexdigest()
+        assert got == fx[key], name
