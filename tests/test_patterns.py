@@ -1,7 +1,7 @@
 """Seeded pattern generators of the native library (host only): the same
 rectangles as the Python generators used to build the benchmark and test
-inputs, the reference generators' lattice geometry for the inline and
-staggered arrays (create_patterns_staggered.c:113-151), and the particle
+inputs, the reference generator's own inline and staggered arrays (fixtures from
+create_patterns_staggered.c:113-162), and the particle
 placement's invariants (create_patterns_random.c:58-138)."""
 import os
 import subprocess
@@ -32,27 +32,38 @@ def _cells(rects):
     return {(i, j) for a, b, c, d in rects for i in range(a, b + 1) for j in range(c, d + 1)}
 
 
-@pytest.mark.parametrize("scale", [1, 2, 4])
-def test_inline_and_staggered_lattices(scale):
-    """Variant 1: obstacles of scale x scale sub-cells on a lattice of pitch
-    (1 + spacing) * scale, the first one centred on the corner (shifted by
-    scale // 2); variant 2: every other lattice column moved by half a
-    row pitch.  Checked against the lattice written out directly."""
-    n_base, sx, sy = 12, 2, 3
-    n = n_base * scale
-    obj, stx, sty, center = scale, (1 + sx) * scale, (1 + sy) * scale, scale // 2
+def _fixture_params():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "patterns_staggered.json")) as f:
+        return json.load(f)
+
+
+def _parse_pattern(text):
+    lines = text.split("\n")
+    n = int(lines[0])
+    rects = [tuple(int(t) for t in ln.split()) for ln in lines[1:1 + n]]
+    assert len(rects) == n
+    return rects
+
+
+@pytest.mark.parametrize("rec", _fixture_params(),
+                         ids=lambda r: f"n{r['n_base']}_s{r['scale']}_x{r['spacing_x']}y{r['spacing_y']}")
+def test_inline_and_staggered_arrays_equal_reference_generator(rec):
+    """Variants 1 and 2 (the inline and the staggered array) equal the
+    reference generator's pattern_1 and pattern_2 line for line: the files
+    create_patterns_staggered.c writes for the same base grid, scale and
+    spacings (fixture written by tests/golden/make_pattern_golden.py from
+    the reference's generator).  With free edge columns, the same minus the
+    rectangles in the first and last i row."""
+    n = rec["n_base"] * rec["scale"]
     for variant in (1, 2):
-        want = set()
-        for r0 in range(-center - 5 * scale, n + 1, sty):
-            for ci, c0 in enumerate(range(-center - 5 * scale, n + 5 * scale + 1, stx)):
-                off = (sty // 2) if (variant == 2 and ((c0 + 5 * scale + center) // stx) % 2) else 0
-                for r in range(r0 + off, r0 + off + obj):
-                    for c in range(c0, c0 + obj):
-                        if 1 <= r <= n - 2 and 0 <= c < n:
-                            want.add((r + 1, c + 1))
-        got = mgfvm.pattern_staggered(n_base, sx, sy, scale, variant, 0)
-        assert _cells(got) == want, (scale, variant)
-        assert got == sorted(got)           # one cell per rectangle, i-major then j
+        want = _parse_pattern(rec[f"pattern_{variant}"])
+        got = mgfvm.pattern_staggered(rec["n_base"], rec["spacing_x"], rec["spacing_y"],
+                                      rec["scale"], variant, 0, False)
+        assert got == want, variant
+        free = mgfvm.pattern_staggered(rec["n_base"], rec["spacing_x"], rec["spacing_y"],
+                                       rec["scale"], variant, 0, True)
+        assert free == [r for r in want if r[0] not in (1, n)], variant
 
 
 def test_perturbed_moves_each_obstacle_at_most_one_unit():
