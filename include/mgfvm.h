@@ -184,6 +184,32 @@ int mgf_reset(mgf_solver *s);
 /* Wait for all work on the solver's stream. */
 int mgf_synchronize(mgf_solver *s);
 
+/* ---- a solve that allocates nothing ------------------------------------ */
+
+/* Size of the static device scratch of mgf_solve_noalloc: one array per
+ * device the library runs on, held until the process ends.  It holds every
+ * level of one problem of up to 256 x 256 cells (FP64 or mixed). */
+#define MGF_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code around graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mgf_create + mgf_solve(p->nvcycle, p->convergence) +
+ * mgf_get_fields(level 0) + mgf_destroy in one call that calls no hipMalloc,
+ * hipFree or other allocating routine.  Every device array of the solve
+ * (all levels' state and scratch, control block, history, launch
+ * descriptors) is carved out of the library's static scratch of
+ * MGF_NOALLOC_SCRATCH_BYTES; a problem that needs more fails with
+ * MGF_ERR_LIMIT before any device work.  V-cycles run eagerly (no graph, one
+ * lane) on `stream` (a hipStream_t; NULL: the default stream) on the current
+ * device (or opt->device).  Outputs are caller-owned host buffers: history
+ * (p->nvcycle entries), the level-0 u, v, p, sc (npx*npy each); any may be
+ * NULL.  Results are bit-identical to the allocating calls.  Every call on
+ * one device shares that device's scratch and overwrites it (it is cleared
+ * where a solve needs zeros): calls that overlap on one device race. */
+int mgf_solve_noalloc(const mgf_problem *p, const mgf_options *opt, void *stream,
+                      double *history, int *ncycles, int *converged, double *u, double *v,
+                      double *pr, double *sc);
+
 /* ---- reference-granular steps (parity and debugging) ------------------ */
 
 /* One SIMPLE relaxation of level ig: fluxes, solve_u, solve_v,

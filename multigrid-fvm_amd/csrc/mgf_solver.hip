@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -86,6 +87,12 @@ struct mgf_solver {
     DevCtl *h_ctl = nullptr;              /* pinned mirror */
     double *d_hist = nullptr;             /* nb x hist_cap */
     int hist_cap = 0;
+    /* a solver of mgf_solve_noalloc: every device array is carved out of
+     * the library's static scratch (scr_cur .. scr_end), the control-block
+     * mirror is ordinary host memory, and the stream is the caller's */
+    int noalloc = 0;
+    char *scr_cur = nullptr, *scr_end = nullptr;
+    int own_stream = 1;
     char err[512] = {0};
 };
 
@@ -121,6 +128,31 @@ static int check_launch(mgf_solver *s)
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+/* Device memory of a solver: hipMalloc, or for a no-alloc solver the next
+ * 256-byte aligned piece of the static scratch (never given back: such a
+ * solver lives for one call). */
+static int dev_alloc(mgf_solver *s, void **p, size_t bytes)
+{
+    if (s->noalloc) {
+        const size_t b = align256(bytes);
+        if ((size_t)(s->scr_end - s->scr_cur) < b)
+            return set_err(s, MGF_ERR_LIMIT,
+                           "the problem needs more device memory than the %zu-byte no-alloc "
+                           "scratch holds",
+                           (size_t)MGF_NOALLOC_SCRATCH_BYTES);
+        *p = s->scr_cur;
+        s->scr_cur += b;
+        return MGF_OK;
+    }
+    HIPCHK(s, hipMalloc(p, bytes));
+    return MGF_OK;
+}
+
+static void dev_free(const mgf_solver *s, void *p)
+{
+    if (p && !s->noalloc) (void)hipFree(p);
+}
 
 /* ---------------------------------------------------------------------- */
 /* arena                                                                    */
@@ -394,10 +426,11 @@ static int build_program(mgf_solver *s, Program &P, const std::vector<int> &memb
     for (auto &g : gm) ndesc += g.size();
     for (auto &p : pf) ndesc += 2 * p.size();
     P.members = members;
-    HIPCHK(s, hipMalloc(&P.d_members, (members.size() + 1) * sizeof(int)));
+    int arc;
+    if ((arc = dev_alloc(s, (void **)&P.d_members, (members.size() + 1) * sizeof(int)))) return arc;
     HIPCHK(s, hipMemcpy(P.d_members, members.data(), members.size() * sizeof(int),
                         hipMemcpyHostToDevice));
-    HIPCHK(s, hipMalloc(&P.d_desc, ndesc * sizeof(DevLevel)));
+    if ((arc = dev_alloc(s, (void **)&P.d_desc, ndesc * sizeof(DevLevel)))) return arc;
     std::vector<DevLevel> host;
     host.reserve(ndesc);
     P.groups.assign(gm.size(), DevGroup());
@@ -456,12 +489,14 @@ static int build_program(mgf_solver *s, Program &P, const std::vector<int> &memb
 static int ensure_hist(mgf_solver *s, int need)
 {
     if (need <= s->hist_cap) return MGF_OK;
-    const int cap = need > 4096 ? need : 4096;
+    const int cap = need > 4096 || s->noalloc ? need : 4096;
     double *h = NULL;
     HIPCHK(s, hipStreamSynchronize(s->st));
-    HIPCHK(s, hipMalloc(&h, (size_t)cap * s->nb * sizeof(double)));
-    HIPCHK(s, hipMemset(h, 0, (size_t)cap * s->nb * sizeof(double)));
-    if (s->d_hist) (void)hipFree(s->d_hist);
+    int rc = dev_alloc(s, (void **)&h, (size_t)cap * s->nb * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(s, hipMemsetAsync(h, 0, (size_t)cap * s->nb * sizeof(double), s->st));
+    HIPCHK(s, hipStreamSynchronize(s->st));
+    dev_free(s, s->d_hist);
     s->d_hist = h;
     s->hist_cap = cap;
     return MGF_OK;
@@ -477,8 +512,22 @@ extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mg
 static bool seeds_supported(const double &) { return false; }
 static bool seeds_supported(const mgf_dual &) { return true; }
 
+/* scratch (or null): build a no-alloc solver whose device arrays are carved
+ * out of [scratch, scratch + scratch_bytes) and which launches on the
+ * caller's stream cst */
+static int create_solver(const mgf_problem *const *probs, int n, const mgf_options *opt,
+                         mgf_solver **out, const mgf_seed *seeds, char *scratch,
+                         size_t scratch_bytes, hipStream_t cst);
+
 int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_options *opt,
                             mgf_solver **out, const mgf_seed *seeds)
+{
+    return create_solver(probs, n, opt, out, seeds, nullptr, 0, nullptr);
+}
+
+static int create_solver(const mgf_problem *const *probs, int n, const mgf_options *opt,
+                         mgf_solver **out, const mgf_seed *seeds, char *scratch,
+                         size_t scratch_bytes, hipStream_t cst)
 {
     if (!probs || n < 1 || !out) return set_err(NULL, MGF_ERR_ARG, "bad arguments");
     const real_t probe_scalar = 0.0;
@@ -498,6 +547,11 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
     mgf_solver *s = new mgf_solver();
     s->opt = o;
     s->nb = n;
+    if (scratch) {
+        s->noalloc = 1;
+        s->scr_cur = scratch;
+        s->scr_end = scratch + scratch_bytes;
+    }
     s->inst.resize(n);
     for (Inst &I : s->inst) {
         I.plan = NULL;
@@ -565,7 +619,10 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
         return set_err(NULL, MGF_ERR_DEVICE, "cannot select device %d", o.device);
     }
     (void)hipGetDevice(&s->device);
-    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    if (s->noalloc) {
+        s->st = cst;
+        s->own_stream = 0;
+    } else if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
         s->st = nullptr;
         mgf_destroy(s);
         return set_err(NULL, MGF_ERR_DEVICE, "stream creation failed");
@@ -578,13 +635,39 @@ int mgf_batch_create_seeded(const mgf_problem *const *probs, int n, const mgf_op
         for (int ig = 0; ig < I.ng; ++ig)
             total += level_bytes(&I.plan->lv[ig], o.precision == MGF_PREC_MIXED);
     s->arena_bytes = total;
-    if (hipMalloc(&s->arena, total) != hipSuccess) {
-        s->arena = nullptr;
-        mgf_destroy(s);
-        return set_err(NULL, MGF_ERR_DEVICE, "device allocation of %zu bytes failed", total);
+    if (s->noalloc) {
+        /* everything a solve of these problems carves, checked before any
+         * device work: the arena, the control blocks, the histories and
+         * (bounded by) the launch descriptors of every level in a group and
+         * a transfer pair */
+        size_t ndesc = 0;
+        int maxnv0 = 1;
+        for (Inst &I : s->inst) {
+            ndesc += 3 * (size_t)I.ng;
+            maxnv0 = I.nvcycle > maxnv0 ? I.nvcycle : maxnv0;
+        }
+        const size_t need = align256(total) + align256((size_t)n * sizeof(DevCtl)) +
+                            align256((size_t)maxnv0 * n * sizeof(double)) +
+                            align256(ndesc * sizeof(DevLevel)) + align256((n + 1) * sizeof(int));
+        if (need > scratch_bytes) {
+            mgf_destroy(s);
+            return set_err(NULL, MGF_ERR_LIMIT,
+                           "the problem needs %zu bytes of device memory; the no-alloc scratch "
+                           "holds %zu",
+                           need, scratch_bytes);
+        }
     }
-    if (hipMalloc(&s->d_ctl, (size_t)n * sizeof(DevCtl)) != hipSuccess ||
-        hipHostMalloc(&s->h_ctl, (size_t)n * sizeof(DevCtl), 0) != hipSuccess) {
+    if ((rc = dev_alloc(s, &s->arena, total)) ||
+        (rc = dev_alloc(s, (void **)&s->d_ctl, (size_t)n * sizeof(DevCtl)))) {
+        char keep[512];
+        snprintf(keep, sizeof keep, "%s", s->err);
+        mgf_destroy(s);
+        return set_err(NULL, rc, "%s", keep);
+    }
+    if (s->noalloc)
+        s->h_ctl = new DevCtl[n];
+    else if (hipHostMalloc(&s->h_ctl, (size_t)n * sizeof(DevCtl), 0) != hipSuccess) {
+        s->h_ctl = nullptr;
         mgf_destroy(s);
         return set_err(NULL, MGF_ERR_DEVICE, "control block allocation failed");
     }
@@ -625,12 +708,103 @@ extern "C" int mgf_create(const mgf_problem *p, const mgf_options *opt, mgf_solv
     return mgf_batch_create(one, 1, opt, out);
 }
 
+/* ---------------------------------------------------------------------- */
+/* the solve that allocates nothing                                         */
+/* ---------------------------------------------------------------------- */
+
+/* The device memory of mgf_solve_noalloc: one static array per device,
+ * held until the process ends.  In host code `scratch` names a host-side
+ * stand-in, not the device array: its device address comes from
+ * hipGetSymbolAddress, once per device with that device current, and only
+ * that address is ever handed to a kernel or a copy. */
+static __device__ __align__(256) unsigned char scratch[MGF_NOALLOC_SCRATCH_BYTES];
+
+#define MGF_NOALLOC_MAX_DEVICES 64
+static void *g_scratch_addr[MGF_NOALLOC_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int scratch_address(int dev, char **out)
+{
+    if (dev < 0 || dev >= MGF_NOALLOC_MAX_DEVICES)
+        return set_err(NULL, MGF_ERR_LIMIT, "device %d beyond the no-alloc device table", dev);
+    pthread_mutex_lock(&g_scratch_mu);
+    int rc = MGF_OK;
+    if (!g_scratch_addr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            rc = set_err(NULL, MGF_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                         hipGetErrorString(e));
+        else
+            g_scratch_addr[dev] = ptr;
+    }
+    *out = (char *)g_scratch_addr[dev];
+    pthread_mutex_unlock(&g_scratch_mu);
+    return rc;
+}
+
+extern "C" int mgf_solve_noalloc(const mgf_problem *p, const mgf_options *opt, void *stream,
+                                 double *history, int *ncycles, int *converged, double *u,
+                                 double *v, double *pr, double *sc)
+{
+    if (!p) return set_err(NULL, MGF_ERR_ARG, "null problem");
+    mgf_options o;
+    if (opt)
+        o = *opt;
+    else
+        mgf_options_default(&o);
+    /* eager V-cycles on one stream: capturing and instantiating a graph, and
+     * lane streams, would allocate */
+    o.use_graph = 0;
+    o.lanes = 1;
+    o.compact = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(NULL, MGF_ERR_DEVICE, "no HIP device available");
+    if (o.device >= 0 && hipSetDevice(o.device) != hipSuccess)
+        return set_err(NULL, MGF_ERR_DEVICE, "cannot select device %d", o.device);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    o.device = dev;
+    char *base = nullptr;
+    int rc = scratch_address(dev, &base);
+    if (rc) return rc;
+    const mgf_problem *one[1] = {p};
+    mgf_solver *s = nullptr;
+    rc = create_solver(one, 1, &o, &s, nullptr, base, MGF_NOALLOC_SCRATCH_BYTES,
+                       (hipStream_t)stream);
+    if (rc) return rc;
+    rc = mgf_solve(s, p->nvcycle, p->convergence, history, ncycles, converged);
+    if (rc == MGF_OK || rc == MGF_ERR_NONFINITE) {
+        const char *names[4] = {"u", "v", "p", "sc"};
+        double *outs[4] = {u, v, pr, sc};
+        for (int k = 0; k < 4; ++k) {
+            if (!outs[k]) continue;
+            const int r2 = mgf_get_field(s, 0, names[k], outs[k]);
+            if (r2) {
+                rc = r2;
+                break;
+            }
+        }
+    }
+    if (rc) {
+        char keep[512];
+        snprintf(keep, sizeof keep, "%s", s->err);
+        (void)hipStreamSynchronize(s->st);
+        mgf_destroy(s);
+        return set_err(NULL, rc, "%s", keep);
+    }
+    (void)hipStreamSynchronize(s->st);
+    mgf_destroy(s);
+    return MGF_OK;
+}
+
 /* the caller has synchronised the stream the program's graph ran on */
-static void release_program(Program &P)
+static void release_program(const mgf_solver *s, Program &P)
 {
     if (P.graph_ready) (void)hipGraphExecDestroy(P.gexec);
-    if (P.d_desc) (void)hipFree(P.d_desc);
-    if (P.d_members) (void)hipFree(P.d_members);
+    dev_free(s, P.d_desc);
+    dev_free(s, P.d_members);
     P = Program();
 }
 
@@ -639,16 +813,21 @@ extern "C" void mgf_destroy(mgf_solver *s)
     if (!s) return;
     for (hipStream_t l : s->lane_st)
         if (l) (void)hipStreamSynchronize(l);
-    for (Program &P : s->lanes) release_program(P);
-    release_program(s->full);
+    for (Program &P : s->lanes) release_program(s, P);
+    release_program(s, s->full);
     for (size_t l = 1; l < s->lane_st.size(); ++l) (void)hipStreamDestroy(s->lane_st[l]);
     for (hipEvent_t e : s->lane_ev)
         if (e) (void)hipEventDestroy(e);
-    if (s->arena) (void)hipFree(s->arena);
-    if (s->d_hist) (void)hipFree(s->d_hist);
-    if (s->d_ctl) (void)hipFree(s->d_ctl);
-    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
-    if (s->st) (void)hipStreamDestroy(s->st);
+    dev_free(s, s->arena);
+    dev_free(s, s->d_hist);
+    dev_free(s, s->d_ctl);
+    if (s->h_ctl) {
+        if (s->noalloc)
+            delete[] s->h_ctl;
+        else
+            (void)hipHostFree(s->h_ctl);
+    }
+    if (s->st && s->own_stream) (void)hipStreamDestroy(s->st);
     for (Inst &I : s->inst) mgf_plan_free(I.plan);
     delete s;
 }
@@ -1037,7 +1216,7 @@ static int ensure_lanes(mgf_solver *s, int nl)
  * (largest first, each to the lightest lane), captured when graphs are on */
 static int build_lanes(mgf_solver *s, const std::vector<int> &alive, int nl)
 {
-    for (Program &P : s->lanes) release_program(P);
+    for (Program &P : s->lanes) release_program(s, P);
     s->lanes.clear();
     std::vector<int> order(alive);
     auto cells = [&](int b) { return (double)s->inst[b].lv[0].npx * s->inst[b].lv[0].npy; };
@@ -1090,7 +1269,7 @@ static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
     if (rc) return rc;
     if ((rc = reset_ctl(s, 0, tol, limit))) return rc;
     HIPCHK(s, hipStreamSynchronize(s->st));
-    for (Program &P : s->lanes) release_program(P);
+    for (Program &P : s->lanes) release_program(s, P);
     s->lanes.clear();
     std::vector<int> alive;
     for (int b = 0; b < s->nb; ++b)

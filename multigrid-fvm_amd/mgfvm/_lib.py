@@ -24,7 +24,7 @@ EXPORTS = [
     "mgf_time_kernel", "mgf_reset", "mgf_synchronize", "mgf_batch_create", "mgf_batch_size",
     "mgf_batch_num_levels", "mgf_batch_level_dims", "mgf_batch_solve", "mgf_batch_history",
     "mgf_batch_get_field", "mgf_pattern_bernoulli", "mgf_pattern_staggered",
-    "mgf_pattern_particles", "mgf_set_inlet_profile",
+    "mgf_pattern_particles", "mgf_set_inlet_profile", "mgf_solve_noalloc",
 ]
 
 PREC_FP64 = 0
@@ -105,6 +105,8 @@ def lib():
         L.mgf_reset.argtypes = [vp]
         L.mgf_synchronize.argtypes = [vp]
         L.mgf_set_inlet_profile.argtypes = [vp, ip, P_d]
+        L.mgf_solve_noalloc.argtypes = [ctypes.POINTER(ProblemStruct), ctypes.POINTER(Options), vp,
+                                        P_d, P_i, P_i, P_d, P_d, P_d, P_d]
         PP = ctypes.POINTER(ProblemStruct)
         L.mgf_batch_create.argtypes = [ctypes.POINTER(PP), ip, ctypes.POINTER(Options),
                                        ctypes.POINTER(vp)]
@@ -420,3 +422,30 @@ def pattern_particles(n_grid: int, ratio: float, pct1: float, pct2: float, pct3:
     """Rectangles of mgf_pattern_particles (the native generator)."""
     return _rects(lib().mgf_pattern_particles, n_grid, ratio, pct1, pct2, pct3, seed,
                   int(free_edge_columns))
+
+
+def solve_noalloc(inputdata: str, pattern: str, precision: int = PREC_FP64, stream=None,
+                  device: int = -1):
+    """mgf_solve_noalloc on the file pair: the whole solve in the library's
+    static device scratch (no device allocation).  Returns (history,
+    ncycles, converged, {"u", "v", "p", "sc": level-0 arrays})."""
+    L = lib()
+    p = read_problem(inputdata, pattern)
+    try:
+        q = p.contents
+        npx, npy = q.nfx + 2, q.nfy + 2
+        opt = Options()
+        L.mgf_options_default(ctypes.byref(opt))
+        opt.precision = precision
+        opt.device = device
+        hist = np.zeros(max(q.nvcycle, 1))
+        out = {k: np.zeros(npx * npy) for k in ("u", "v", "p", "sc")}
+        nc, cv = ctypes.c_int(0), ctypes.c_int(0)
+        rc = L.mgf_solve_noalloc(p, ctypes.byref(opt), stream, _dptr(hist), ctypes.byref(nc),
+                                 ctypes.byref(cv), _dptr(out["u"]), _dptr(out["v"]),
+                                 _dptr(out["p"]), _dptr(out["sc"]))
+        if rc != 0:
+            raise MgfError(rc, L.mgf_error_string().decode())
+        return hist[:nc.value], nc.value, bool(cv.value), out
+    finally:
+        free_problem(p)
