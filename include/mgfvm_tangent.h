@@ -62,24 +62,12 @@ int mgf_tangent_objectives(mgf_solver *s, int b, double J[2], double dJ[2]);
  * mass-consistent change.  Outputs: J[2]; g_inlet[2 * nfy] with
  * dJ[k]/d(inlet row r) at k * nfy + (r - 1); g_amuf[2]; g_prsc[2];
  * *converged = 1 when every direction's solve met its tolerance (any output
- * may be NULL).  This is the gradient entry the reference's design lists as
- * an adjoint call (the reference has no derivative capability); here it is
- * computed in forward mode, nfy + 2 solves batched on the device. */
+ * may be NULL).  Forward mode: nfy + 2 solves batched on the device; the
+ * reverse-mode gradient of one objective is mgf_adjoint (mgfvm_adjoint.h,
+ * libmgfvm.so), which the tests check against this. */
 int mgf_tangent_gradient(const mgf_problem *p, const mgf_options *opt, int max_batch,
                          double J[2], double *g_inlet, double g_amuf[2], double g_prsc[2],
                          int *converged);
-
-/* The gradient call of the reference design's boundary, under its name and
- * arguments: dJ/d(inlet row r) in g_inlet[r - 1] (r = 1..nfy), dJ/d(amuf)
- * in *g_amuf and dJ/d(prsc) in *g_prsc, for J the objective's value
- * (objective 0: the pressure drop J1, 1: the outlet scalar flux J2; as in
- * mgf_tangent_objectives) at the solution of the problem solver s was
- * created with (from its initial state, with its options; s itself is not
- * changed).  Computed as mgf_tangent_gradient computes it: forward mode,
- * the nfy + 2 directions in one batch, not a reverse sweep.  The reference
- * has no derivative capability, so this replaces no reference routine.
- * Any output may be NULL; MGF_ERR_ARG for another objective. */
-int mgf_adjoint(mgf_solver *s, int objective, double *g_inlet, double *g_amuf, double *g_prsc);
 
 #ifdef __cplusplus
 }

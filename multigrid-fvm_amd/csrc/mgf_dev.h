@@ -148,6 +148,9 @@ int mgf_field_array(mgf_solver *s, int b, int ig, const char *name, real_t **ptr
 const mgf_problem *mgf_instance_problem(mgf_solver *s, int b);
 const mgf_options *mgf_solver_options(mgf_solver *s);
 int mgf_set_error(mgf_solver *s, int code, const char *msg);
+const DevLevel *mgf_instance_levels(mgf_solver *s, int b, int *ng);
+void **mgf_adjoint_slot(mgf_solver *s);
+void mgf_adjoint_release(void *work);
 
 /* launch wrappers (mgf_kernels.hip) */
 void mgf_launch_fluxes(const DevGroup &G, hipStream_t st);

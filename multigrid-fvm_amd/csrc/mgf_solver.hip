@@ -93,6 +93,7 @@ struct mgf_solver {
     int noalloc = 0;
     char *scr_cur = nullptr, *scr_end = nullptr;
     int own_stream = 1;
+    void *adj = nullptr;                  /* adjoint workspace (mgf_adjoint.hip) */
     char err[512] = {0};
 };
 
@@ -827,6 +828,9 @@ extern "C" void mgf_destroy(mgf_solver *s)
         else
             (void)hipHostFree(s->h_ctl);
     }
+#ifndef MGF_TANGENT
+    if (s->adj) mgf_adjoint_release(s->adj);
+#endif
     if (s->st && s->own_stream) (void)hipStreamDestroy(s->st);
     for (Inst &I : s->inst) mgf_plan_free(I.plan);
     delete s;
@@ -1433,6 +1437,16 @@ int mgf_field_array(mgf_solver *s, int b, int ig, const char *name, real_t **ptr
     *st = s->st;
     return MGF_OK;
 }
+
+/* host copies of problem b's level descriptors (for the adjoint) */
+const DevLevel *mgf_instance_levels(mgf_solver *s, int b, int *ng)
+{
+    if (!s || b < 0 || b >= s->nb) return NULL;
+    *ng = s->inst[b].ng;
+    return s->inst[b].lv;
+}
+
+void **mgf_adjoint_slot(mgf_solver *s) { return &s->adj; }
 
 const mgf_problem *mgf_instance_problem(mgf_solver *s, int b)
 {

@@ -143,30 +143,6 @@ extern "C" int mgf_tangent_gradient(const mgf_problem *p, const mgf_options *opt
     return rc;
 }
 
-extern "C" int mgf_adjoint(mgf_solver *s, int objective, double *g_inlet, double *g_amuf,
-                           double *g_prsc)
-{
-    if (!s) {
-        mgf_set_error(NULL, MGF_ERR_ARG, "null solver");
-        return MGF_ERR_ARG;
-    }
-    if (objective != 0 && objective != 1)
-        return mgf_set_error(s, MGF_ERR_ARG, "objective must be 0 (pressure drop) or 1 (outlet flux)");
-    const mgf_problem *p = mgf_instance_problem(s, 0);
-    const mgf_options *opt = mgf_solver_options(s);
-    if (!p || !opt) return mgf_set_error(s, MGF_ERR_ARG, "solver holds no problem");
-    const int nfy = p->nfy;
-    std::vector<double> gi(2 * (size_t)nfy);
-    double ga[2], gp[2];
-    const int rc = mgf_tangent_gradient(p, opt, 0, NULL, gi.data(), ga, gp, NULL);
-    if (rc) return mgf_set_error(s, rc, mgf_error_string());
-    if (g_inlet)
-        for (int r = 0; r < nfy; ++r) g_inlet[r] = gi[(size_t)objective * nfy + r];
-    if (g_amuf) *g_amuf = ga[objective];
-    if (g_prsc) *g_prsc = gp[objective];
-    return MGF_OK;
-}
-
 /* ---- mixed-precision launchers: not part of this library --------------- */
 
 static void not_here(const char *what)

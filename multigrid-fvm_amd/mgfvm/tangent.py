@@ -21,7 +21,7 @@ from ._lib import PKG_DIR, InletProfile, MgfError, Options, ProblemStruct
 TAN_PATH = os.path.join(PKG_DIR, "libmgfvm_tangent.so")
 
 EXPORTS = ["mgf_tangent_batch_create", "mgf_tangent_get_field", "mgf_tangent_objectives",
-           "mgf_tangent_gradient", "mgf_adjoint"]
+           "mgf_tangent_gradient"]
 
 
 class Seed(ctypes.Structure):
@@ -58,7 +58,6 @@ def tlib():
         L.mgf_tangent_gradient.argtypes = [PP, ctypes.POINTER(Options), ip, P_d, P_d, P_d, P_d,
                                            P_i]
         L.mgf_create.argtypes = [PP, ctypes.POINTER(Options), ctypes.POINTER(vp)]
-        L.mgf_adjoint.argtypes = [vp, ip, P_d, P_d, P_d]
         _tlib = L
     return _tlib
 
@@ -210,37 +209,6 @@ def gradient_c(inputdata: str, pattern: str, inlet_u=None, max_batch: int = 0,
         return (np.array(J[:]), np.array(gi[:]).reshape(2, nfy), np.array(ga[:]),
                 np.array(gp[:]), bool(conv.value))
     finally:
-        L.mgf_problem_free(p)
-
-
-def adjoint_c(inputdata: str, pattern: str, objective: int, inlet_u=None, device: int = -1):
-    """mgf_adjoint on a solver of the sensitivity library created for the
-    problem: the gradient of objective 0 (pressure drop) or 1 (outlet scalar
-    flux).  Returns (dJ_dinlet [nfy], dJ_damuf, dJ_dprsc)."""
-    L = tlib()
-    p = ctypes.POINTER(ProblemStruct)()
-    rc = L.mgf_problem_read(inputdata.encode(), pattern.encode(), ctypes.byref(p))
-    if rc != 0:
-        raise MgfError(rc, L.mgf_error_string().decode())
-    s = ctypes.c_void_p()
-    try:
-        nfy = p.contents.nfy
-        opt = Options()
-        L.mgf_options_default(ctypes.byref(opt))
-        opt.device = device
-        with InletProfile(p, inlet_u):
-            rc = L.mgf_create(p, ctypes.byref(opt), ctypes.byref(s))
-        if rc != 0:
-            raise MgfError(rc, L.mgf_error_string().decode())
-        gi = (ctypes.c_double * nfy)()
-        ga, gp = ctypes.c_double(), ctypes.c_double()
-        rc = L.mgf_adjoint(s, objective, gi, ctypes.byref(ga), ctypes.byref(gp))
-        if rc != 0:
-            raise MgfError(rc, L.mgf_last_error(s).decode())
-        return np.array(gi[:]), ga.value, gp.value
-    finally:
-        if s:
-            L.mgf_destroy(s)
         L.mgf_problem_free(p)
 
 

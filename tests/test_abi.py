@@ -34,6 +34,16 @@ def test_header_declares_and_library_exports_everything():
     assert sorted(set(mgfvm.EXPORTS)) == names
 
 
+def test_adjoint_header_matches_binding():
+    """include/mgfvm_adjoint.h's entry points are exported by libmgfvm.so
+    and declared by the Python binding."""
+    from mgfvm import adjoint
+    names = _header_names("mgfvm_adjoint.h")
+    assert sorted(adjoint.EXPORTS) == names
+    L = mgfvm.lib()
+    assert all(hasattr(L, n) for n in names)
+
+
 def test_abi_version():
     assert mgfvm.lib().mgf_abi_version() == 1
 
@@ -168,7 +178,7 @@ def _header_names(*headers):
 
 
 @pytest.mark.parametrize("lib,headers", [
-    ("libmgfvm.so", ("mgfvm.h",)),
+    ("libmgfvm.so", ("mgfvm.h", "mgfvm_adjoint.h")),
     ("libmgfvm_tangent.so", ("mgfvm.h", "mgfvm_tangent.h")),
 ])
 def test_library_exports_only_its_c_abi(lib, headers):

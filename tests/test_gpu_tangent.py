@@ -200,24 +200,3 @@ def test_c_gradient_entry_matches_batched_directions(tmp_path):
         assert got[4], mb
         for a, b in zip(got[:4], ref[:4]):
             assert np.array_equal(np.asarray(a), np.asarray(b)), mb
-
-
-def test_adjoint_entry_matches_gradient(tmp_path):
-    """mgf_adjoint (the boundary's gradient call, per objective, on a solver
-    handle) returns the bits of mgf_tangent_gradient's row for that
-    objective."""
-    from mgfvm.tangent import adjoint_c, gradient_c
-    nf = 32
-    inlet = _inlet(nf, seed=9)
-    prob = Problem(nfx=nf, nfy=nf, ngrid=3, ig_obs=3, nvcycle=3000, convergence=1e-12,
-                   isolve=2, nswpm=2, nswpp=6, amuf=0.02,
-                   bc_xp=(float(inlet.mean()), 0.0, 0.0, 0.0),
-                   obstacles=random_mask(8, 0.85, 6))
-    pin, ppat = prob.write(str(tmp_path / "a"))
-    _say("[gpu] mgf_adjoint vs mgf_tangent_gradient")
-    J, gi, ga, gp, conv = gradient_c(pin, ppat, inlet_u=inlet)
-    assert conv
-    for obj in (0, 1):
-        ai, aa, ap = adjoint_c(pin, ppat, obj, inlet_u=inlet)
-        assert np.array_equal(ai, gi[obj]), obj
-        assert aa == ga[obj] and ap == gp[obj], obj
