@@ -45,6 +45,9 @@ typedef struct mgf_adjoint_opts {
     double rtol;       /* stop when ||dJ/dx - K^T lambda|| <= rtol ||dJ/dx|| (default 1e-11) */
     int max_iter;      /* GMRES iterations in all (default 600) */
     int restart;       /* Krylov vectors kept before a restart (default 40) */
+    int only_converged;/* 1 (default): problems whose last solve did not meet its
+                          tolerance are skipped (gradients NaN, iters -1): the
+                          adjoint is that of the steady solution */
 } mgf_adjoint_opts;
 
 void mgf_adjoint_opts_default(mgf_adjoint_opts *o);
@@ -59,15 +62,18 @@ int mgf_objectives(mgf_solver *s, int b, double J[2]);
  * change), dJ/d(amuf) in *g_amuf, dJ/d(prsc) in *g_prsc.  Any output may
  * be NULL.  The adjoint call of the design's boundary: one reverse solve
  * per objective.  Works on FP64 and mixed-precision solvers (the state is
- * FP64 in both; the adjoint runs in FP64).  MGF_ERR_NONFINITE if the solve
- * did not reach its tolerance within max_iter (outputs then hold the last
+ * FP64 in both; the adjoint runs in FP64).  MGF_ERR_CONFIG if the solver's
+ * last solve did not converge; MGF_ERR_NONFINITE if the adjoint solve did
+ * not reach its tolerance within max_iter (outputs then hold the last
  * iterate's gradient). */
 int mgf_adjoint(mgf_solver *s, int objective, double *g_inlet, double *g_amuf, double *g_prsc);
 
 /* Every problem of a batch at once.  Outputs (any may be NULL), problem b
  * at index b or, for g_inlet, at offset sum_{k<b} nfy_k: J[b],
- * g_inlet, g_amuf[b], g_prsc[b], iters[b] (GMRES iterations),
- * relres[b] (final ||dJ/dx - K^T lambda|| / ||dJ/dx||). */
+ * g_inlet, g_amuf[b], g_prsc[b], iters[b] (GMRES iterations, -1 for a
+ * skipped problem), relres[b] (final ||dJ/dx - K^T lambda|| / ||dJ/dx||).
+ * MGF_ERR_NONFINITE if a problem that was not skipped missed its
+ * tolerance. */
 int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint_opts *opt, double *J,
                       double *g_inlet, double *g_amuf, double *g_prsc, int *iters,
                       double *relres);

@@ -532,11 +532,19 @@ __global__ void k_adj_defect(const AdjLevel *As)
 }
 
 /* one colour of red-black on the pressure-correction operator of coefp
- * (V6:1030-1040: areax^2 / apu, areay^2 / apv, obstacle cells 1e30) */
+ * (V6:1030-1040: areax^2 / apu, areay^2 / apv, obstacle cells 1e30).  A
+ * fluid cell all of whose faces are fixed (a pocket enclosed by obstacles,
+ * on this level) has a diagonal of ~1e-33: its multiplier is not
+ * determined by the equations (no active face carries it), and a sweep
+ * there would amplify the defect by 1e33, so it keeps dp = 0. */
+#define ADJ_FIXED_FACE 1.0e29
 __global__ void k_adj_rb_p(const AdjLevel *As, int colour)
 {
     ADJ_LEVEL
     if (((i + j) & 1) != colour || !crow(L, i, j)) return;
+    if (A.apu[ij - 1] >= ADJ_FIXED_FACE && A.apu[ij] >= ADJ_FIXED_FACE &&
+        A.apv[ij - nx] >= ADJ_FIXED_FACE && A.apv[ij] >= ADJ_FIXED_FACE)
+        return;
     const double aw = L.axax / A.apu[ij - 1], ae = L.axax / A.apu[ij];
     const double as = L.ayay / A.apv[ij - nx], an = L.ayay / A.apv[ij];
     const double ap = ((aw + ae) + as) + an;
@@ -1356,6 +1364,7 @@ extern "C" void mgf_adjoint_opts_default(mgf_adjoint_opts *o)
     o->rtol = 1e-11;
     o->max_iter = 600;
     o->restart = 40;
+    o->only_converged = 1;
 }
 
 /* Givens rotation of (a, b) */
@@ -1411,13 +1420,19 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
         for (int b = 0; b < nb; ++b)
             AHIP(s, hipMemsetAsync(w->vec[b].X, 0, w->vec[b].n4 * sizeof(double), st));
     }
-    std::vector<int> done(nb, 0), total(nb, 0);
+    std::vector<int> done(nb, 0), total(nb, 0), skip(nb, 0);
     std::vector<double> rel(nb, 1.0);
-    for (int b = 0; b < nb; ++b)
-        if (!(gnorm[b] > 0.0)) {
-            done[b] = 1;      /* dJ/dx = 0: lambda = 0 */
+    for (int b = 0; b < nb; ++b) {
+        if (o.only_converged) {
+            const int cv = mgf_instance_converged(s, b);
+            if (cv < 0) return cv;
+            skip[b] = !cv;
+        }
+        if (skip[b] || !(gnorm[b] > 0.0)) {
+            done[b] = 1;      /* skipped, or dJ/dx = 0: lambda = 0 */
             rel[b] = 0.0;
         }
+    }
     std::vector<double> H((size_t)nb * (m + 1) * m), cs((size_t)nb * m), sn((size_t)nb * m);
     std::vector<double> gv((size_t)nb * (m + 1)), yv((size_t)nb * m);
     std::vector<int> jcount(nb), inner(nb);
@@ -1565,6 +1580,10 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
     AHIP(s, hipStreamSynchronize(st));
     int unconverged = 0;
     for (int b = 0; b < nb; ++b) {
+        if (skip[b]) {
+            rel[b] = NAN;
+            continue;
+        }
         rel[b] = gnorm[b] > 0.0 ? sqrt(hh[b]) / gnorm[b] : 0.0;
         if (!std::isfinite(rel[b])) nonfinite = 1;
         if (!(rel[b] <= 10.0 * o.rtol)) unconverged = 1;
@@ -1627,12 +1646,14 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
             ga += part[((size_t)b * maxblk + q) * 2];
             gp += part[((size_t)b * maxblk + q) * 2 + 1];
         }
+        const double nan = skip[b] ? NAN : 0.0;
         if (J) J[b] = hJ[2 * b + objective];
-        if (g_amuf) g_amuf[b] = ga;
-        if (g_prsc) g_prsc[b] = gp;
+        if (g_amuf) g_amuf[b] = ga + nan;
+        if (g_prsc) g_prsc[b] = gp + nan;
         if (g_inlet)
-            for (int r = 0; r < nfy; ++r) g_inlet[off[b] + r] = gin[off[b] + r] + guxp / (double)nfy;
-        if (iters) iters[b] = total[b];
+            for (int r = 0; r < nfy; ++r)
+                g_inlet[off[b] + r] = gin[off[b] + r] + guxp / (double)nfy + nan;
+        if (iters) iters[b] = skip[b] ? -1 : total[b];
         if (relres) relres[b] = rel[b];
     }
     if (nonfinite) return mgf_set_error(s, MGF_ERR_NONFINITE, "non-finite adjoint residual");
@@ -1648,6 +1669,12 @@ extern "C" int mgf_adjoint(mgf_solver *s, int objective, double *g_inlet, double
     if (!s) return mgf_set_error(s, MGF_ERR_ARG, "null solver");
     if (mgf_batch_size(s) != 1)
         return mgf_set_error(s, MGF_ERR_UNSUPPORTED, "mgf_adjoint needs a single-problem solver");
+    const int cv = mgf_instance_converged(s, 0);
+    if (cv < 0) return cv;
+    if (!cv)
+        return mgf_set_error(s, MGF_ERR_CONFIG,
+                             "the solver's last solve did not converge: the adjoint is that of "
+                             "the steady solution");
     return mgf_batch_adjoint(s, objective, nullptr, nullptr, g_inlet, g_amuf, g_prsc, nullptr,
                              nullptr);
 }

@@ -151,6 +151,7 @@ int mgf_set_error(mgf_solver *s, int code, const char *msg);
 const DevLevel *mgf_instance_levels(mgf_solver *s, int b, int *ng);
 void **mgf_adjoint_slot(mgf_solver *s);
 void mgf_adjoint_release(void *work);
+int mgf_instance_converged(mgf_solver *s, int b);
 
 /* launch wrappers (mgf_kernels.hip) */
 void mgf_launch_fluxes(const DevGroup &G, hipStream_t st);

@@ -1348,6 +1348,19 @@ static int last_value(mgf_solver *s, int b, double *out)
     return MGF_OK;
 }
 
+/* 1 when problem b's last solve met its tolerance with finite residuals,
+ * 0 when it did not (or none ran), negative on a device error */
+int mgf_instance_converged(mgf_solver *s, int b)
+{
+    if (!s || b < 0 || b >= s->nb) return MGF_ERR_ARG;
+    int rc = read_ctl(s);
+    if (rc) return rc;
+    double last = 0.0;
+    if ((rc = last_value(s, b, &last))) return rc;
+    const DevCtl &c = s->h_ctl[b];
+    return (c.cyc > 0 && last < c.tol && !c.nonfinite) ? 1 : 0;
+}
+
 extern "C" int mgf_solve(mgf_solver *s, int nvcycle, double tol, double *history, int *ncycles,
                          int *converged)
 {

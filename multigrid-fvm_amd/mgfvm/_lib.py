@@ -345,6 +345,12 @@ class Batch:
         except Exception:
             pass
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
     def reset(self):
         self._check(lib().mgf_reset(self.h))
 

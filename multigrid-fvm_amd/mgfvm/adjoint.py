@@ -27,7 +27,8 @@ OBJ_OUTLET_FLUX = 1
 
 
 class AdjointOpts(ctypes.Structure):
-    _fields_ = [("rtol", ctypes.c_double), ("max_iter", ctypes.c_int), ("restart", ctypes.c_int)]
+    _fields_ = [("rtol", ctypes.c_double), ("max_iter", ctypes.c_int), ("restart", ctypes.c_int),
+                ("only_converged", ctypes.c_int)]
 
 
 _bound = False
@@ -58,7 +59,7 @@ def _check(h, rc, allow_nonfinite=False):
 
 
 def opts(rtol: Optional[float] = None, max_iter: Optional[int] = None,
-         restart: Optional[int] = None) -> AdjointOpts:
+         restart: Optional[int] = None, only_converged: Optional[bool] = None) -> AdjointOpts:
     o = AdjointOpts()
     _bind().mgf_adjoint_opts_default(ctypes.byref(o))
     if rtol is not None:
@@ -67,6 +68,8 @@ def opts(rtol: Optional[float] = None, max_iter: Optional[int] = None,
         o.max_iter = max_iter
     if restart is not None:
         o.restart = restart
+    if only_converged is not None:
+        o.only_converged = int(only_converged)
     return o
 
 

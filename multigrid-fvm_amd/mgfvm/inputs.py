@@ -325,3 +325,24 @@ def staggered_batch(count: int = 256, seed0: int = 1000):
         out.append(Problem(nfx=n, nfy=n, ngrid=ngrid, ig_obs=ig_obs, nvcycle=2999, nswpm=2,
                            nswpp=6, isolve=2, amuf=1.0 / re, obstacles=rects))
     return out
+
+
+def adjoint_batch(count: int = 64, seed0: int = 4000, convergence: float = 1e-8,
+                  n_base: int = 16):
+    """The batched-adjoint workload: 512x512 channels, 6 levels (coarsest
+    16x16), each with a Bernoulli mask on an n_base x n_base base grid
+    (obstacles on the coarsest level, ig_obs = 6 for the 16x16 base) whose
+    porosity is drawn uniformly from [0.7, 0.95], Re drawn uniformly from
+    [20, 300], red-black 2/6, plug inlet and outlet (u = 1), the primal
+    converged to the FP64 sum |div| < `convergence` (the adjoint is taken at
+    the converged state).  Instance k uses seed seed0 + k for every draw."""
+    out = []
+    for k in range(count):
+        rng = Rng(seed0 + k)
+        por = 0.7 + 0.25 * rng.uniform()
+        re = 20.0 + 280.0 * rng.uniform()
+        ig_obs = 6 - int(round(math.log2(n_base // 16))) if n_base >= 16 else 6
+        out.append(Problem(nfx=512, nfy=512, ngrid=6, ig_obs=ig_obs, nvcycle=2999,
+                           convergence=convergence, nswpm=2, nswpp=6, isolve=2, amuf=1.0 / re,
+                           obstacles=random_mask(n_base, por, seed0 + k)))
+    return out

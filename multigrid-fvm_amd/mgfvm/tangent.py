@@ -233,10 +233,14 @@ def steady_objectives(inlet_u, amuf, prsc, problem, device: int = -1):
 
 
 def torch_function():
-    """The torch.autograd.Function over steady_objectives: forward(inlet_u
-    [nfy], amuf, prsc, problem) -> J [2] (float64); backward returns the
-    vector-Jacobian products from the forward-mode Jacobian computed in the
-    same solve."""
+    """The forward-mode torch.autograd.Function over steady_objectives:
+    forward(inlet_u [nfy], amuf, prsc, problem) -> J [2] (float64, on
+    inlet_u's device) computes the whole Jacobian in the same batched solve
+    (nfy + 2 directions); backward returns its vector-Jacobian products for
+    the tensor inputs that need them (None for float hyperparameters), each
+    on its input's device and dtype.  The reverse-mode Function, which
+    solves only the primal in forward and one adjoint per backward, is
+    mgfvm.adjoint.torch_function."""
     import torch
 
     class SteadyObjectives(torch.autograd.Function):
@@ -248,14 +252,20 @@ def torch_function():
                 raise RuntimeError("steady solve did not converge")
             ctx.save_for_backward(torch.as_tensor(d_in), torch.as_tensor(d_mu),
                                   torch.as_tensor(d_pr))
-            ctx.devices = (inlet_u.device,)
-            return torch.as_tensor(J, dtype=torch.float64)
+            ctx.like = (inlet_u, amuf if torch.is_tensor(amuf) else None,
+                        prsc if torch.is_tensor(prsc) else None)
+            return torch.as_tensor(J, dtype=torch.float64).to(inlet_u.device)
 
         @staticmethod
         def backward(ctx, g):
             d_in, d_mu, d_pr = ctx.saved_tensors
-            g = g.to(torch.float64).cpu()
-            return ((g @ d_in).to(ctx.devices[0]), (g @ d_mu).reshape(()),
-                    (g @ d_pr).reshape(()), None)
+            g = g.detach().to("cpu", torch.float64)
+            outs = [None, None, None, None]
+            for k, (d, t) in enumerate(zip((d_in, d_mu, d_pr), ctx.like)):
+                if t is None or not ctx.needs_input_grad[k]:
+                    continue
+                v = g @ d
+                outs[k] = (v if k == 0 else v.reshape(())).to(t.dtype).to(t.device)
+            return tuple(outs)
 
     return SteadyObjectives

@@ -14,6 +14,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
+def pytest_sessionstart(session):
+    """Initialise torch's GPU runtime before any test loads the solver
+    library: torch carries its own HIP runtime, and it finds no device when
+    the library's runtime has initialised the GPU first in the process."""
+    try:
+        import torch
+        torch.cuda.is_available()
+    except Exception:
+        pass
+
+
 def gpu_available() -> bool:
     try:
         import torch

@@ -177,6 +177,13 @@ def test_torch_function_backward(tmp_path):
     np.testing.assert_allclose(x.grad.numpy(), w @ din, rtol=1e-14)
     np.testing.assert_allclose(mu.grad.item(), w @ dmu, rtol=1e-14)
     np.testing.assert_allclose(pr.grad.item(), w @ dpr, rtol=1e-14)
+    # float hyperparameters get no gradient slot; a device inlet keeps its device
+    xd = torch.tensor(inlet, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    Jd = F.apply(xd, prob.amuf, prob.prsc, prob)
+    assert Jd.device == xd.device
+    Jd[0].backward()
+    assert xd.grad.device == xd.device
+    np.testing.assert_allclose(xd.grad.cpu().numpy(), din[0], rtol=1e-14)
 
 
 def test_c_gradient_entry_matches_batched_directions(tmp_path):
