@@ -61,6 +61,14 @@ HBM_PEAK_GBS = 8000.0
 # `bench.py --roofline-only --reps 10`, each scaled by what the counter reads
 # for a 1 GiB device copy, read from the committed summary of those passes
 TRAFFIC_PROFILE = os.path.join("profiles", "traffic_relax_uvs_512x84.md")
+# launch shapes and lane count of the batch, measured by the library's
+# autotuner (multigrid-fvm_amd/tuning/make_table.py --workload batch256)
+TUNING_TABLE = os.path.join("multigrid-fvm_amd", "tuning", "batch256_mi355x.txt")
+
+
+def tuning_table():
+    path = os.path.join(ROOT, TUNING_TABLE)
+    return (open(path).read(), TUNING_TABLE) if os.path.exists(path) else (None, None)
 
 
 def profiled_traffic():
@@ -352,8 +360,9 @@ def main():
             return
 
         # ---- batched mixed precision (the measured line) ----------------
+        table, table_src = tuning_table()
         bt = mgfvm.Batch(pairs, precision=mgfvm.PREC_MIXED, device=0,
-                         use_graph=not args.no_graph)
+                         use_graph=not args.no_graph, tuning=table)
         for _ in range(args.warmup):
             bt.reset()
             bt.solve()
@@ -374,7 +383,7 @@ def main():
         conv_only = None
         if sum(conv) < len(probs):
             keep = [pairs[k] for k in range(len(probs)) if conv[k]]
-            bc = mgfvm.Batch(keep, precision=mgfvm.PREC_MIXED, device=0)
+            bc = mgfvm.Batch(keep, precision=mgfvm.PREC_MIXED, device=0, tuning=table)
             bc.solve()
             bc.reset()
             _sync(bc)
@@ -590,6 +599,7 @@ def main():
             "ms_per_step": round(1e3 * elapsed / args.steps, 2), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
             "config": {"workload": WORKLOAD, "problems": len(probs),
+                       "launch_tuning": table_src,
                        "converged_per_step": sum(conv),
                        "v_cycles_mean": round(sum(ncyc) / len(ncyc), 1),
                        "v_cycles_max": max(ncyc),

@@ -112,8 +112,10 @@ typedef struct mgf_options {
                                   this many groups ("lanes", balanced by cell count), each
                                   replaying its own V-cycle on its own stream, so one
                                   lane's latency-bound coarse levels and launch tails
-                                  overlap another lane's work (1-4; 0: automatic, 3).
-                                  Result-neutral. */
+                                  overlap another lane's work (1-4; 0: automatic, 3, or
+                                  the tuning table's).  Result-neutral. */
+    const char *tuning;        /* launch-shape table written by mgf_autotune (text; NULL:
+                                  the built-in rules).  Result-neutral. */
 } mgf_options;
 
 typedef struct mgf_solver mgf_solver;
@@ -276,6 +278,26 @@ int mgf_get_fields(mgf_solver *s, int ig, double *u, double *v, double *p,
 int mgf_set_inlet_profile(mgf_solver *s, int b, const double *inlet_u);
 
 /* ---- measurement ------------------------------------------------------------ */
+
+/* Project-written autotuner (no counterpart in the reference).  For a
+ * mixed-precision red-black solver with the fused relaxation, measures on
+ * the device, with HIP events, the result-neutral launch shapes of the
+ * merged relaxation on every level shape of its problems (the u, v, sc
+ * launch: 16-row tiles with 512 threads, or 32-row tiles with 256 / 512
+ * threads; the one-launch p': 16-row tiles with 1024 threads, or 32-row
+ * tiles with 512), each candidate `reps` times over all the problems'
+ * levels of that shape, and the lane count (1-4, unless the options fix it)
+ * by timing the whole solve of the batch with each (the lanes matter in
+ * the tail, where only the slowest problems still run); applies the
+ * fastest and writes the table as text
+ * into table (cap bytes, NUL-terminated; may be NULL), in the format
+ * mgf_options.tuning reads:
+ *     lanes <n>
+ *     level <npx> <npy> <uvs 1|2> <p' 1|2> <threads>
+ * Returns the table's length, or a negative status.  The problems are
+ * reset to their initial state afterwards.  MGF_ERR_UNSUPPORTED for other
+ * solvers (whose launches have no alternative shapes). */
+int mgf_autotune(mgf_solver *s, int reps, char *table, size_t cap);
 
 /* Kernel ids for mgf_time_kernel */
 #define MGF_KERNEL_SWEEP_PP  0   /* one full smoother sweep of the p' equation (FP64

@@ -127,6 +127,11 @@ struct DevGroup {
     int any_finest;          /* some member is its instance's level 0 */
     int depth;               /* levels above the coarsest */
     int up;                  /* 1: the up-leg subset (no finest levels) */
+    /* launch shapes of the merged mixed relaxation (result-neutral), from the
+     * solver's tuning table: 0 the built-in size rule, 1 the small-level
+     * shape (16-row tiles), 2 the large-level shape (32-row tiles); nt_asm:
+     * threads of the large u, v, sc launch (0: the options') */
+    int uvs_shape, pp_shape, nt_asm;
 };
 
 /* a restriction / prolongation between two same-length groups */
@@ -246,6 +251,8 @@ int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int n
 /* its first launch alone (u, v and sc; for timing: it rewrites what it
  * wrote before, given unchanged inputs); 0 when not applicable */
 int mgf_launch_uvs_mixed(const DevGroup &G, int isolve, int nswm, SweepCfg cfg, hipStream_t st);
+/* its one-launch p' (for timing, as above); 0 when not applicable */
+int mgf_launch_pp_merged_mixed(const DevGroup &G, int nswp, hipStream_t st);
 void mgf_launch_smooth_mixed(const DevGroup &G, int field, int isolve, int nsw, SweepCfg cfg,
                              hipStream_t st);
 /* the FP32 sweeps alone (they only touch the correction buffers); returns

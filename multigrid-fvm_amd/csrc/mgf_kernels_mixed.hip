@@ -751,7 +751,9 @@ void mgf_launch_relax_field_mixed(const DevGroup &G, int field, int isolve, int 
 template <int S>
 static void launch_uvs_s(const DevGroup &G, int nt, hipStream_t st)
 {
-    if (level_at_most(G, MGF_UVS_SMALL_MAX)) {
+    if (G.nt_asm) nt = G.nt_asm;
+    const bool small = G.uvs_shape ? G.uvs_shape == 1 : level_at_most(G, MGF_UVS_SMALL_MAX);
+    if (small) {
         /* small levels: latency-bound, shaped by their own tile rows and
          * threads */
         const dim3 g0 = tile_grid(G.npx - 2, G.npy - 2, G.count, MGF_UVS_SMALL_ROWS);
@@ -792,7 +794,8 @@ static void launch_pp_merged_s(const DevGroup &G, hipStream_t st)
 {
     int nx, ny;
     sweep_extents(G, 2, nx, ny);
-    if (level_at_most(G, MGF_PP_SMALL_MAX)) {
+    const bool small = G.pp_shape ? G.pp_shape == 1 : level_at_most(G, MGF_PP_SMALL_MAX);
+    if (small) {
         const dim3 g = tile_grid(nx - 1, ny - 1, G.count, MGF_SMALL_TILE_ROWS);
         hipLaunchKernelGGL((k_rb32x<S, 1024, MODE_ASM_CORR, 2, MGF_SMALL_TILE_ROWS>), g,
                            dim3(BX, 16), 0, st, G.d, 2, 0, 3);
@@ -803,12 +806,9 @@ static void launch_pp_merged_s(const DevGroup &G, hipStream_t st)
     }
 }
 
-int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
-                                  SweepCfg cfg, int err, const DevPair *R, hipStream_t st)
+int mgf_launch_pp_merged_mixed(const DevGroup &G, int nswp, hipStream_t st)
 {
-    if (nswp < 1 || nswp > MGF_MAX_SWEEPS_PER_LAUNCH ||
-        !mgf_launch_uvs_mixed(G, isolve, nswm, cfg, st))
-        return 0;
+    if (nswp < 1 || nswp > MGF_MAX_SWEEPS_PER_LAUNCH) return 0;
     switch (nswp) {
     case 1: launch_pp_merged_s<1>(G, st); break;
     case 2: launch_pp_merged_s<2>(G, st); break;
@@ -817,6 +817,16 @@ int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int n
     case 5: launch_pp_merged_s<5>(G, st); break;
     default: launch_pp_merged_s<6>(G, st); break;
     }
+    return 1;
+}
+
+int mgf_launch_relax_merged_mixed(const DevGroup &G, int isolve, int nswm, int nswp,
+                                  SweepCfg cfg, int err, const DevPair *R, hipStream_t st)
+{
+    if (nswp < 1 || nswp > MGF_MAX_SWEEPS_PER_LAUNCH ||
+        !mgf_launch_uvs_mixed(G, isolve, nswm, cfg, st))
+        return 0;
+    (void)mgf_launch_pp_merged_mixed(G, nswp, st);
     if (err) mgf_launch_reduce_err(G, err == 2, st);
     if (R && !G.any_pbound_x && !G.any_pbound_y) {
         mgf_launch_update_restv(G, *R, 2, st);

@@ -28,6 +28,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <string>
 #include <vector>
 
 #include "mgf_dev.h"
@@ -65,6 +67,12 @@ struct Program {
     int graph_ready = 0;
 };
 
+/* one line of the tuning table: the merged relaxation's shapes on levels
+ * of npx x npy points */
+struct TuneEntry {
+    int npx, npy, uvs, pp, nt;
+};
+
 struct mgf_solver {
     int nb = 0;
     std::vector<Inst> inst;
@@ -94,6 +102,8 @@ struct mgf_solver {
     char *scr_cur = nullptr, *scr_end = nullptr;
     int own_stream = 1;
     void *adj = nullptr;                  /* adjoint workspace (mgf_adjoint.hip) */
+    std::vector<TuneEntry> tune;          /* launch shapes per level shape */
+    int tune_lanes = 0;                   /* lane count of the tuning table (0: none) */
     char err[512] = {0};
 };
 
@@ -339,11 +349,18 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
 /* groups and the V-cycle program                                           */
 /* ---------------------------------------------------------------------- */
 
-static void fill_group_meta(DevGroup &G, const std::vector<DevLevel> &m)
+static void fill_group_meta(const mgf_solver *s, DevGroup &G, const std::vector<DevLevel> &m)
 {
     G.count = (int)m.size();
     G.npx = m[0].npx;
     G.npy = m[0].npy;
+    G.uvs_shape = G.pp_shape = G.nt_asm = 0;
+    for (const TuneEntry &t : s->tune)
+        if (t.npx == G.npx && t.npy == G.npy) {
+            G.uvs_shape = t.uvs;
+            G.pp_shape = t.pp;
+            G.nt_asm = t.nt;
+        }
     G.max_npcopy = 0;
     G.any_pbound_x = G.any_pbound_y = G.any_outflow = 0;
     G.any_finest = 0;
@@ -443,7 +460,7 @@ static int build_program(mgf_solver *s, Program &P, const std::vector<int> &memb
     for (size_t g = 0; g < gm.size(); ++g) {
         DevGroup &G = P.groups[g];
         G.d = P.d_desc + host.size();
-        fill_group_meta(G, gm[g]);
+        fill_group_meta(s, G, gm[g]);
         host.insert(host.end(), gm[g].begin(), gm[g].end());
     }
     for (size_t p = 0; p < pf.size(); ++p) {
@@ -512,6 +529,46 @@ extern "C" int mgf_batch_create(const mgf_problem *const *probs, int n, const mg
 /* the derivative seeds exist only where the scalar carries derivatives */
 static bool seeds_supported(const double &) { return false; }
 static bool seeds_supported(const mgf_dual &) { return true; }
+
+/* the tuning table's text (mgf_autotune's format): "lanes N" and
+ * "level NPX NPY UVS PP THREADS" lines, '#' comments */
+static int parse_tuning(mgf_solver *s, const char *text)
+{
+    s->tune.clear();
+    s->tune_lanes = 0;
+    const char *p = text;
+    int line = 0;
+    while (*p) {
+        const char *e = strchr(p, '\n');
+        const size_t len = e ? (size_t)(e - p) : strlen(p);
+        char buf[256];
+        const size_t n = len < sizeof buf - 1 ? len : sizeof buf - 1;
+        memcpy(buf, p, n);
+        buf[n] = 0;
+        ++line;
+        char *hash = strchr(buf, '#');
+        if (hash) *hash = 0;
+        TuneEntry t;
+        int nl;
+        char word[16] = "";
+        if (sscanf(buf, "%15s", word) == 1) {
+            if (!strcmp(word, "lanes") && sscanf(buf, "%*s %d", &nl) == 1 && nl >= 1 &&
+                nl <= 4) {
+                s->tune_lanes = nl;
+            } else if (!strcmp(word, "level") &&
+                       sscanf(buf, "%*s %d %d %d %d %d", &t.npx, &t.npy, &t.uvs, &t.pp, &t.nt) ==
+                           5 &&
+                       t.uvs >= 0 && t.uvs <= 2 && t.pp >= 0 && t.pp <= 2 &&
+                       (t.nt == 0 || t.nt == 256 || t.nt == 512 || t.nt == 1024)) {
+                s->tune.push_back(t);
+            } else {
+                return set_err(s, MGF_ERR_PARSE, "tuning table line %d not understood", line);
+            }
+        }
+        p = e ? e + 1 : p + len;
+    }
+    return MGF_OK;
+}
 
 /* scratch (or null): build a no-alloc solver whose device arrays are carved
  * out of [scratch, scratch + scratch_bytes) and which launches on the
@@ -610,6 +667,13 @@ static int create_solver(const mgf_problem *const *probs, int n, const mgf_optio
             }
         s->smoother = 3;
     }
+    if (o.tuning && (rc = parse_tuning(s, o.tuning))) {
+        char keep[512];
+        snprintf(keep, sizeof keep, "%s", s->err);
+        mgf_destroy(s);
+        return set_err(NULL, rc, "%s", keep);
+    }
+    s->opt.tuning = nullptr;              /* the caller's text need not outlive the call */
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         mgf_destroy(s);
@@ -1196,7 +1260,7 @@ static int capture_graph(mgf_solver *s, Program &P)
 #define MGF_AUTO_LANES 3
 static int lane_count(const mgf_solver *s, size_t n)
 {
-    int nl = s->opt.lanes > 0 ? s->opt.lanes : MGF_AUTO_LANES;
+    int nl = s->opt.lanes > 0 ? s->opt.lanes : (s->tune_lanes > 0 ? s->tune_lanes : MGF_AUTO_LANES);
     if (nl > MGF_MAX_LANES) nl = MGF_MAX_LANES;
     if ((size_t)nl > n) nl = (int)n;
     return nl < 1 ? 1 : nl;
@@ -1624,4 +1688,159 @@ extern "C" int mgf_time_kernel(mgf_solver *s, int kernel, int ig, int reps, doub
     if (bytes_per_launch)
         *bytes_per_launch = fused ? fused_bpu * nunk : (double)spl * 8.0 * (mixed ? 4.0 : 8.0) * nunk;
     return MGF_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* autotuner                                                                */
+/* ---------------------------------------------------------------------- */
+
+/* the shapes of the tuning table on an existing program's groups (its
+ * captured graph, made with the old shapes, is dropped) */
+static void apply_tuning(mgf_solver *s, Program &P)
+{
+    for (DevGroup &G : P.groups) {
+        G.uvs_shape = G.pp_shape = G.nt_asm = 0;
+        for (const TuneEntry &t : s->tune)
+            if (t.npx == G.npx && t.npy == G.npy) {
+                G.uvs_shape = t.uvs;
+                G.pp_shape = t.pp;
+                G.nt_asm = t.nt;
+            }
+    }
+    if (P.graph_ready) {
+        (void)hipGraphExecDestroy(P.gexec);
+        P.graph_ready = 0;
+    }
+}
+
+extern "C" int mgf_autotune(mgf_solver *s, int reps, char *table, size_t cap)
+{
+    if (!s || reps < 1) return set_err(s, MGF_ERR_ARG, "bad arguments");
+#ifdef MGF_TANGENT
+    (void)table;
+    (void)cap;
+    return set_err(s, MGF_ERR_UNSUPPORTED, "the sensitivity library has no alternative shapes");
+#else
+    if (s->opt.precision != MGF_PREC_MIXED || s->isolve != 2 || s->opt.split_assembly)
+        return set_err(s, MGF_ERR_UNSUPPORTED,
+                       "launch shapes are tuned for mixed-precision red-black with fused assembly");
+    int rc;
+    /* a representative state: three V-cycles from the initial one */
+    if ((rc = mgf_reset(s))) return rc;
+    {
+        std::vector<int> lim(s->nb, 3);
+        std::vector<double> tol(s->nb, 0.0);
+        if ((rc = run_cycles(s, lim.data(), tol.data()))) return rc;
+    }
+    for (int b = 0; b < s->nb; ++b) s->h_ctl[b].done = 0;
+    HIPCHK(s, hipMemcpyAsync(s->d_ctl, s->h_ctl, (size_t)s->nb * sizeof(DevCtl),
+                             hipMemcpyHostToDevice, s->st));
+    const SweepCfg fcfg = mgf_sweep_cfg(s->opt.sweeps_per_launch, s->opt.sweep_threads, 0);
+    hipEvent_t e0, e1;
+    HIPCHK(s, hipEventCreate(&e0));
+    HIPCHK(s, hipEventCreate(&e1));
+    auto timed = [&](auto &&launch) {
+        launch();
+        launch();
+        (void)hipEventRecord(e0, s->st);
+        for (int r = 0; r < reps; ++r) launch();
+        (void)hipEventRecord(e1, s->st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return (double)ms / reps;
+    };
+    std::vector<TuneEntry> tab;
+    for (const DevGroup &G : s->full.groups) {
+        if (G.up) continue;
+        bool seen = false;
+        for (const TuneEntry &t : tab) seen |= t.npx == G.npx && t.npy == G.npy;
+        if (seen) continue;
+        DevGroup Gc = G;
+        Gc.uvs_shape = 1;
+        Gc.nt_asm = 0;
+        if (!mgf_launch_uvs_mixed(Gc, s->isolve, s->nswpm, fcfg, s->st)) continue;
+        /* the u, v, sc launch: 16-row / 512, 32-row / 256, 32-row / 512 */
+        const int cu[3][2] = {{1, 512}, {2, 256}, {2, 512}};
+        int bu = 0;
+        double tu = 1e30;
+        for (int k = 0; k < 3; ++k) {
+            Gc.uvs_shape = cu[k][0];
+            Gc.nt_asm = cu[k][1];
+            const double t = timed([&] { (void)mgf_launch_uvs_mixed(Gc, s->isolve, s->nswpm, fcfg, s->st); });
+            if (t < tu) {
+                tu = t;
+                bu = k;
+            }
+        }
+        Gc.uvs_shape = cu[bu][0];
+        Gc.nt_asm = cu[bu][1];
+        int bp = 1;
+        double tp = 1e30;
+        for (int k = 1; k <= 2 && s->nswpp >= 1 && s->nswpp <= MGF_MAX_SWEEPS_PER_LAUNCH; ++k) {
+            Gc.pp_shape = k;
+            const double t = timed([&] { (void)mgf_launch_pp_merged_mixed(Gc, s->nswpp, s->st); });
+            if (t < tp) {
+                tp = t;
+                bp = k;
+            }
+        }
+        tab.push_back(TuneEntry{G.npx, G.npy, cu[bu][0], bp, cu[bu][1]});
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if ((rc = check_launch(s))) return rc;
+    s->tune = tab;
+    apply_tuning(s, s->full);
+    /* lanes: the whole solve of the batch (each problem to its own stopping
+     * test) with each count; the lanes matter most in the tail, where only
+     * the slowest problems still run, so nothing shorter measures them (the
+     * caller's explicit choice stands) */
+    int best_lanes = 0;
+    if (s->opt.lanes == 0 && s->nb > 1) {
+        double best = 1e30;
+        std::vector<int> lim(s->nb);
+        std::vector<double> tol(s->nb);
+        for (int b = 0; b < s->nb; ++b) {
+            lim[b] = s->inst[b].nvcycle;
+            tol[b] = s->inst[b].tol;
+        }
+        for (int nl = 1; nl <= MGF_MAX_LANES && nl <= s->nb; ++nl) {
+            s->tune_lanes = nl;
+            double t = 1e30;
+            for (int rep = 0; rep < 1; ++rep) {
+                if ((rc = mgf_reset(s))) return rc;
+                HIPCHK(s, hipStreamSynchronize(s->st));
+                const auto t0 = std::chrono::steady_clock::now();
+                if ((rc = run_cycles(s, lim.data(), tol.data()))) return rc;
+                const double dt =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                t = dt < t ? dt : t;
+            }
+            if (t < best) {
+                best = t;
+                best_lanes = nl;
+            }
+        }
+    }
+    s->tune_lanes = best_lanes;
+    if ((rc = mgf_reset(s))) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->st));
+    std::string text = "# mgf_autotune: merged mixed relaxation shapes per level shape\n";
+    char line[128];
+    if (best_lanes) {
+        snprintf(line, sizeof line, "lanes %d\n", best_lanes);
+        text += line;
+    }
+    for (const TuneEntry &t : tab) {
+        snprintf(line, sizeof line, "level %d %d %d %d %d\n", t.npx, t.npy, t.uvs, t.pp, t.nt);
+        text += line;
+    }
+    if (table && cap > 0) {
+        const size_t n = text.size() < cap - 1 ? text.size() : cap - 1;
+        memcpy(table, text.data(), n);
+        table[n] = 0;
+    }
+    return (int)text.size();
+#endif
 }

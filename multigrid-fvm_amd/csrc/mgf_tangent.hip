@@ -182,6 +182,12 @@ int mgf_launch_uvs_mixed(const DevGroup &, int, int, SweepCfg, hipStream_t)
     return 0;
 }
 
+int mgf_launch_pp_merged_mixed(const DevGroup &, int, hipStream_t)
+{
+    not_here("mgf_launch_pp_merged_mixed");
+    return 0;
+}
+
 int mgf_launch_sweeps_mixed(const DevGroup &, int, int, int, SweepCfg, hipStream_t)
 {
     not_here("mgf_launch_sweeps_mixed");

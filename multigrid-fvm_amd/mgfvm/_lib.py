@@ -24,7 +24,7 @@ EXPORTS = [
     "mgf_time_kernel", "mgf_reset", "mgf_synchronize", "mgf_batch_create", "mgf_batch_size",
     "mgf_batch_num_levels", "mgf_batch_level_dims", "mgf_batch_solve", "mgf_batch_history",
     "mgf_batch_get_field", "mgf_pattern_bernoulli", "mgf_pattern_staggered",
-    "mgf_pattern_particles", "mgf_set_inlet_profile", "mgf_solve_noalloc",
+    "mgf_pattern_particles", "mgf_set_inlet_profile", "mgf_solve_noalloc", "mgf_autotune",
 ]
 
 PREC_FP64 = 0
@@ -47,7 +47,8 @@ class Options(ctypes.Structure):
                 ("device", ctypes.c_int), ("strict_limits", ctypes.c_int),
                 ("sweeps_per_launch", ctypes.c_int), ("sweep_threads", ctypes.c_int),
                 ("split_assembly", ctypes.c_int), ("gs_order", ctypes.c_int),
-                ("compact", ctypes.c_int), ("lanes", ctypes.c_int)]
+                ("compact", ctypes.c_int), ("lanes", ctypes.c_int),
+                ("tuning", ctypes.c_char_p)]
 
 
 class ProblemStruct(ctypes.Structure):
@@ -102,6 +103,7 @@ def lib():
         L.mgf_get_field.argtypes = [vp, ip, cp, P_d]
         L.mgf_set_field.argtypes = [vp, ip, cp, P_d]
         L.mgf_time_kernel.argtypes = [vp, ip, ip, ip, P_d, P_d]
+        L.mgf_autotune.argtypes = [vp, ip, ctypes.c_char_p, ctypes.c_size_t]
         L.mgf_reset.argtypes = [vp]
         L.mgf_synchronize.argtypes = [vp]
         L.mgf_set_inlet_profile.argtypes = [vp, ip, P_d]
@@ -304,9 +306,10 @@ class Batch:
     def __init__(self, pairs: Sequence[Tuple[str, str]], precision: int = PREC_FP64,
                  use_graph: bool = True, device: int = -1, sweeps_per_launch: int = 0,
                  sweep_threads: int = 0, split_assembly: bool = False, compact: bool = True,
-                 gs_lex: bool = False, lanes: int = 0):
+                 gs_lex: bool = False, lanes: int = 0, tuning: Optional[str] = None):
         L = lib()
         probs = [read_problem(a, b) for a, b in pairs]
+        tuning_b = tuning.encode() if tuning else None
         try:
             arr = (ctypes.POINTER(ProblemStruct) * len(probs))(*probs)
             opt = Options()
@@ -320,6 +323,7 @@ class Batch:
             opt.compact = 1 if compact else 0
             opt.gs_order = 1 if gs_lex else 0
             opt.lanes = lanes
+            opt.tuning = tuning_b
             h = ctypes.c_void_p()
             rc = L.mgf_batch_create(arr, len(probs), ctypes.byref(opt), ctypes.byref(h))
             if rc != 0:
@@ -350,6 +354,15 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+    def autotune(self, reps: int = 20) -> str:
+        """mgf_autotune: measure and apply the launch shapes; returns the
+        table (the text mgf_options.tuning / tuning= reads)."""
+        buf = ctypes.create_string_buffer(8192)
+        n = lib().mgf_autotune(self.h, reps, buf, len(buf))
+        if n < 0:
+            self._check(n)
+        return buf.value.decode()
 
     def reset(self):
         self._check(lib().mgf_reset(self.h))

@@ -217,6 +217,16 @@ def test_create_rejects_options(tmp_path, optkw, code):
     assert msg
 
 
+def test_tuning_table_errors(tmp_path):
+    """A malformed tuning table is rejected with MGF_ERR_PARSE before any
+    device work."""
+    prob = channel_128_inline().replace(isolve=2)
+    rc, msg = _create(prob, tmp_path, tuning=b"level 130 130 7 1 512\n")
+    assert rc == -3 and "tuning" in msg
+    rc, msg = _create(prob, tmp_path, tuning=b"speed 11\n")
+    assert rc == -3
+
+
 def test_lexicographic_variant_row_limit(tmp_path):
     prob = Problem(nfx=64, nfy=2048, ngrid=2, ig_obs=2, isolve=2, obstacles=[])
     rc, msg = _create(prob, tmp_path, gs_order=1)
