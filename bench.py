@@ -301,6 +301,47 @@ def roofline_only(args):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def adjoint_figure(count, work):
+    """The batched adjoint: `count` 512^2 problems (random masks, Re in
+    [20, 300]) solved in mixed precision to FP64 sum|div| < 1e-8, then the
+    reverse-mode dJ1/d(inlet profile, amuf) of every converged one in one
+    batched solve; both timed after an untimed set-up run."""
+    import mgfvm
+    from mgfvm import adjoint as gadj
+    from mgfvm.inputs import adjoint_batch
+    aprobs = adjoint_batch(count)
+    apairs = [p.write(os.path.join(work, f"a{k}")) for k, p in enumerate(aprobs)]
+    ba = mgfvm.Batch(apairs, precision=mgfvm.PREC_MIXED, device=0)
+    ba.solve()
+    gadj.batch_adjoint(ba, 0, allow_unconverged=True)      # set-up, untimed
+    ba.reset()
+    _sync(ba)
+    t0 = time.perf_counter()
+    anc, acv = ba.solve()
+    _sync(ba)
+    t_ap = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = gadj.batch_adjoint(ba, 0, allow_unconverged=True)
+    t_aa = time.perf_counter() - t0
+    ba.close()
+    its = [r[4] for r in res if r[4] >= 0]
+    ok = sum(1 for r in res if r[4] >= 0 and r[5] <= 1e-10)
+    print(f"[bench] batched adjoint: primal {t_ap:.2f}s ({sum(acv)} converged), adjoint "
+          f"{t_aa:.2f}s ({ok} gradients)", file=sys.stderr, flush=True)
+    return {
+        "workload": f"{len(aprobs)} channels of 512x512, 6 levels, Bernoulli masks on a "
+                    "16x16 base with porosity U[0.7, 0.95], Re U[20, 300], seeds 4000+k; "
+                    "mixed-precision primal to FP64 sum|div| < 1e-8, then dJ1/d(inlet "
+                    "profile, amuf) of every converged problem in one batched reverse "
+                    "solve (FP64, GMRES to 1e-11 relative residual)",
+        "primal_s": round(t_ap, 3), "primal_converged": int(sum(acv)),
+        "adjoint_s": round(t_aa, 3), "adjoint_converged": ok,
+        "adjoint_over_primal": round(t_aa / t_ap, 3),
+        "gmres_iterations_mean": round(sum(its) / max(len(its), 1), 1),
+        "gmres_iterations_max": max(its) if its else None,
+        "gradients_per_s": round(ok / t_aa, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +364,8 @@ def main():
                     help="launch the batched V-cycles eagerly instead of replaying a hipGraph")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant kernel and a 1 GiB copy (for counter profiles)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the batched-adjoint figure (for kernel traces of that path)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="solve only the batch in FP64, once (for kernel traces of that path)")
     args = ap.parse_args()
@@ -342,6 +385,9 @@ def main():
         sys.exit(1)
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_figure(args.adjoint_count, tempfile.mkdtemp(prefix="mgf_adj_"))))
         return
     torch.cuda.set_device(0)
     work = tempfile.mkdtemp(prefix="mgf_bench_")
@@ -464,6 +510,34 @@ def main():
                         "launch: an effective per-sweep rate, not HBM bandwidth")
                 s.close()
 
+        # ---- the lexicographic Gauss-Seidel variant ------------------------
+        lexv = None
+        if not args.no_single:
+            from mgfvm.inputs import channel_128_inline
+            lexv = {"what": "FP64 solves with isolve 2 as the reference's red-black (the parity "
+                            "path) and as the labelled lexicographic variant (anti-diagonal "
+                            "wavefronts, one workgroup per problem)"}
+            cases = {"128^2 inline array, Re 100, 2/4": channel_128_inline().replace(
+                         isolve=2, nvcycle=2999),
+                     "256^2 staggered array (batch problem 1001), 2/6": next(
+                         p for p in probs if p.nfx == 256 and p.ig_obs == p.ngrid)}
+            for name, pq in cases.items():
+                lin, lpat = pq.write(os.path.join(work, "lex" + str(len(lexv))))
+                row = {}
+                for tag, lex in (("red_black", False), ("lexicographic", True)):
+                    s = mgfvm.Solver(lin, lpat, device=0, gs_lex=lex)
+                    s.solve()
+                    s.reset()
+                    _sync(s)
+                    t0 = time.perf_counter()
+                    _, n, c = s.solve()
+                    _sync(s)
+                    dt = time.perf_counter() - t0
+                    s.close()
+                    row[tag] = {"seconds": round(dt, 4), "v_cycles": n, "converged": bool(c)}
+                lexv[name] = row
+            print(f"[bench] lexicographic variant: {lexv}", file=sys.stderr, flush=True)
+
         # ---- gradient of the 256^2 differentiation workload -----------------
         gradient = None
         if not args.no_gradient:
@@ -516,39 +590,7 @@ def main():
         # ---- batched adjoint: 64 x 512^2 ------------------------------------
         batched_adjoint = None
         if not args.no_gradient and args.adjoint_count > 0:
-            from mgfvm import adjoint as gadj
-            from mgfvm.inputs import adjoint_batch
-            aprobs = adjoint_batch(args.adjoint_count)
-            apairs = [p.write(os.path.join(work, f"a{k}")) for k, p in enumerate(aprobs)]
-            ba = mgfvm.Batch(apairs, precision=mgfvm.PREC_MIXED, device=0)
-            ba.solve()
-            gadj.batch_adjoint(ba, 0, allow_unconverged=True)      # set-up, untimed
-            ba.reset()
-            _sync(ba)
-            t0 = time.perf_counter()
-            anc, acv = ba.solve()
-            _sync(ba)
-            t_ap = time.perf_counter() - t0
-            t0 = time.perf_counter()
-            res = gadj.batch_adjoint(ba, 0, allow_unconverged=True)
-            t_aa = time.perf_counter() - t0
-            ba.close()
-            its = [r[4] for r in res if r[4] >= 0]
-            ok = sum(1 for r in res if r[4] >= 0 and r[5] <= 1e-10)
-            batched_adjoint = {
-                "workload": f"{len(aprobs)} channels of 512x512, 6 levels, Bernoulli masks on a "
-                            "16x16 base with porosity U[0.7, 0.95], Re U[20, 300], seeds 4000+k; "
-                            "mixed-precision primal to FP64 sum|div| < 1e-8, then dJ1/d(inlet "
-                            "profile, amuf) of every converged problem in one batched reverse "
-                            "solve (FP64, GMRES to 1e-11 relative residual)",
-                "primal_s": round(t_ap, 3), "primal_converged": int(sum(acv)),
-                "adjoint_s": round(t_aa, 3), "adjoint_converged": ok,
-                "adjoint_over_primal": round(t_aa / t_ap, 3),
-                "gmres_iterations_mean": round(sum(its) / max(len(its), 1), 1),
-                "gmres_iterations_max": max(its) if its else None,
-                "gradients_per_s": round(ok / t_aa, 2)}
-            print(f"[bench] batched adjoint: primal {t_ap:.2f}s ({sum(acv)} converged), adjoint "
-                  f"{t_aa:.2f}s ({ok} gradients)", file=sys.stderr, flush=True)
+            batched_adjoint = adjoint_figure(args.adjoint_count, work)
 
         # ---- CPU baseline ---------------------------------------------------
         cpu = None
@@ -612,6 +654,7 @@ def main():
                        "single_1024": single,
                        "gradient_256": gradient,
                        "batched_adjoint_512": batched_adjoint,
+                       "lexicographic_variant": lexv,
                        "converging_only": conv_only},
             "roofline": roof, "cpu_baseline": cpu,
         }
