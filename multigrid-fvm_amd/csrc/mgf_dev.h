@@ -38,6 +38,7 @@ struct DevCtl {
 
 struct DevPhys {
     double relxm, relxm1, relxp, sc_obs;
+    double irelxm;        /* 1 / relxm, for the mixed path's assembly */
     int ibc[4];
 };
 

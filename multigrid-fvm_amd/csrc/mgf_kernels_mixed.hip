@@ -5,7 +5,9 @@
  *
  * u, v, sc (defect correction): the assembly kernel evaluates the FP64
  * residual r = su + sum(a_nb phi_nb) - ap phi of the reference's equation
- * and stores the coefficients, 1/ap and r in FP32; nsw FP32 sweeps of the
+ * and stores the coefficients, 1/ap (an FP32 reciprocal) and r in FP32; the
+ * under-relaxed diagonal is ap * (1/relxm), whose rounding the relaxation
+ * terms cancel at the fixed point; nsw FP32 sweeps of the
  * same smoother (red-black or Jacobi) on A d = r from d = 0, with d = 0
  * outside the swept range; then phi += d in FP64.  In exact arithmetic this
  * is the same iteration as nsw sweeps on A phi = b from phi (the smoothers
@@ -13,7 +15,10 @@
  * rounding of d does not limit the converged FP64 state.
  * p': already a correction that starts from zero every relaxation
  * (masserror sets pp = 0, V6:1116), smoothed directly in FP32 with the FP64
- * mass defect -div as right-hand side; pp is then widened to FP64.
+ * mass defect -div as right-hand side; pp is then widened to FP64.  Its
+ * coefficients area^2 / ap (coefp) feed nothing but the FP32 sweeps and are
+ * formed in FP32; at the fixed point the defect, and with it p', is zero
+ * whatever they are.
  * The level-0 sum |div|, the update of u, v, p and every grid transfer stay
  * FP64.
  *
@@ -43,6 +48,25 @@ __device__ __forceinline__ void store_pt(const DevLevelG &L, int g, const Pt32 &
     L.an32[g] = q.an;
     L.iap32[g] = q.iap;
     L.r32[g] = q.r;
+}
+
+/* 1 / a in FP32: v_rcp_f32 (1 ulp) and one Newton step, r + r (1 - a r),
+ * which leaves it within about half an ulp; 0 and infinity keep the
+ * reciprocal's own inf and 0 (the Newton step would turn them into NaN). */
+__device__ __forceinline__ float rcp32(float a)
+{
+    const float r = __builtin_amdgcn_rcpf(a);
+    const float q = __builtin_fmaf(r, __builtin_fmaf(-a, r, 1.0f), r);
+    return __builtin_isfinite(q) ? q : r;
+}
+
+/* The FP32 reciprocal of a diagonal for the correction sweeps (isolve 1
+ * keeps the Jacobi solver's 1e-30 guard).  Only the FP32 correction
+ * operator uses it: the FP64 residual, not these digits, sets the converged
+ * state. */
+__device__ __forceinline__ float rcp_diag(float ap, int isolve)
+{
+    return rcp32(isolve == 1 ? ap + 1.0e-30f : ap);
 }
 
 /* FP64 assembly of one equation at one point plus its FP64 residual, rounded
@@ -76,14 +100,14 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int
     if (field == 0) {
         c = coef_pt<INL>(L, 0, i, j, 0);
         su = L.ax * (L.p[ij] - L.p[ij + 1]);
-        ap = c.ap / P.relxm;
+        ap = c.ap * P.irelxm;
         su = ((su + (P.relxm1 * ap) * L.u[ij]) + L.resu[ij]) + L.bodyx;
         ap = ap + ((L.flags[ij] & MGF_F_SPU) ? 1.0e30 : 0.0);
         phi = L.u;
     } else if (field == 1) {
         c = coef_pt<INL>(L, 1, i, j, 0);
         su = L.ay * (L.p[ij] - L.p[ij + nx]);
-        ap = c.ap / P.relxm;
+        ap = c.ap * P.irelxm;
         su = ((su + (P.relxm1 * ap) * L.v[ij]) + L.resv[ij]) + L.bodyy;
         ap = ap + ((L.flags[ij] & MGF_F_SPV) ? 1.0e30 : 0.0);
         phi = L.v;
@@ -98,8 +122,8 @@ __device__ __forceinline__ Pt32 asm_pt(const DevLevelG &L, int field, int i, int
     const double r = ((((su + c.aw * phi[ij - 1]) + c.ae * phi[ij + 1]) + c.as * phi[ij - nx]) +
                       c.an * phi[ij + nx]) - ap * phi[ij];
     ap_out = ap;
-    return Pt32{(float)c.aw, (float)c.ae, (float)c.as, (float)c.an,
-                (float)(L.isolve == 1 ? 1.0 / (ap + 1.0e-30) : 1.0 / ap), (float)r};
+    return Pt32{(float)c.aw, (float)c.ae, (float)c.as, (float)c.an, rcp_diag((float)ap, L.isolve),
+                (float)r};
 }
 
 /* masserror + coefp in FP64 (V6:1018-1125) at one cell, rounded to FP32;
@@ -132,18 +156,20 @@ __device__ __forceinline__ Pt32 asm_pt_p(const DevLevelG &L, int i, int j, doubl
     double div = ((L.ax * ue - L.ax * uw) + L.ay * vn) - L.ay * vs;
     double su = -div;
     mag = fabs(div);
-    double aw = L.axax / L.apu[ij - 1];
-    double ae = L.axax / L.apu[ij];
-    double as = L.ayay / L.apv[ijs];
-    double an = L.ayay / L.apv[ij];
-    double ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30 : 0.0);
-    const double diag = 1.0e30;
+    /* the p' equation is only ever solved in FP32, so its coefficients
+     * (coefp's area^2 / ap of the faces) are formed in FP32 */
+    const float axax = (float)L.axax, ayay = (float)L.ayay;
+    const float aw = axax * rcp32((float)L.apu[ij - 1]);
+    const float ae = axax * rcp32((float)L.apu[ij]);
+    const float as = ayay * rcp32((float)L.apv[ijs]);
+    const float an = ayay * rcp32((float)L.apv[ij]);
+    float ap = (((aw + ae) + as) + an) + ((L.flags[ij] & MGF_F_OBS) ? 1.0e30f : 0.0f);
+    const float diag = 1.0e30f;
     if (P.ibc[0] == 2 && i == 1) { ap = ap + diag; su = 0.0; }
     if (P.ibc[1] == 2 && i == nx - 2) { ap = ap + diag; su = 0.0; }
     if (P.ibc[2] == 2 && j == 1) { ap = ap + diag; su = 0.0; }
     if (P.ibc[3] == 2 && j == L.npy - 2) { ap = ap + diag; su = 0.0; }
-    return Pt32{(float)aw, (float)ae, (float)as, (float)an,
-                (float)(L.isolve == 1 ? 1.0 / (ap + 1.0e-30) : 1.0 / ap), (float)su};
+    return Pt32{aw, ae, as, an, rcp_diag(ap, L.isolve), (float)su};
 }
 
 /* assembly of u, v or sc over the swept range, stored for the sweeps */

@@ -255,6 +255,7 @@ static int upload_instance(mgf_solver *s, Inst &I, char *&cur, DevCtl *ctl, cons
     DevPhys P;
     P.relxm = c->relxm;
     P.relxm1 = 1.0 - c->relxm;
+    P.irelxm = 1.0 / c->relxm;
     P.relxp = c->relxp;
     P.sc_obs = c->sc_obs;
     for (int k = 0; k < 4; ++k) P.ibc[k] = c->ibc[k];
