@@ -816,57 +816,145 @@ __global__ void k_adj_grad_bc(const AdjLevel *As, const double *const *lamv, int
 
 /* ---- Krylov vector kernels (per problem: blockIdx.z) ------------------ */
 
-/* partial dot products per block of 256 * 8 entries: mode 0 <V_k, W> for
- * k < nk, mode 1 <W, W>, mode 2 <G, G> (nk = 1) */
+/* Dot products of long vectors: every workgroup of DOT_T threads takes a
+ * chunk of DOT_T * DOT_PER entries (thread t: entries t, t + DOT_T, ...),
+ * writes one partial sum per product, and k_dots_sum adds a product's
+ * partials.  Every sum runs in a fixed order (a thread's entries in turn, a
+ * wave's shuffle tree, the four waves in turn, then the same for the
+ * partials), so the results do not depend on scheduling. */
 #define DOT_T 256
 #define DOT_PER 8
-__global__ void k_dots(const AdjVec *Vs, const int *live, int nk, int mode, double *partial,
-                       int nblk)
+/* Krylov vectors per workgroup of k_dots: W is read once per DOT_KB of them */
+#define DOT_KB 16
+
+/* the sums over the workgroup of v[0..NV-1], in thread i < NV of the
+ * returned LDS row out (red: 4 x NV doubles) */
+template <int NV>
+__device__ __forceinline__ void wg_sums(double (&v)[NV], double *red)
 {
-    const AdjVec &W = Vs[blockIdx.z];
-    const int k = blockIdx.y;
-    double acc = 0.0;
-    if (live[blockIdx.z] && k < nk) {
-        const double *a = mode == 0 ? W.V + (size_t)k * W.n4 : (mode == 1 ? W.W : W.G);
-        const double *b = mode == 2 ? W.G : W.W;
-        const long base = (long)blockIdx.x * DOT_T * DOT_PER + threadIdx.x;
-        for (int t = 0; t < DOT_PER; ++t) {
-            const long q = base + (long)t * DOT_T;
-            if (q < W.n4) acc += a[q] * b[q];
-        }
-    }
-    __shared__ double red[DOT_T];
-    red[threadIdx.x] = acc;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[wv * NV + i] = v[i];
     __syncthreads();
-    for (int s = DOT_T / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        partial[((size_t)blockIdx.z * gridDim.y + k) * nblk + blockIdx.x] = red[0];
 }
 
-/* out[b * nk + k] = sum of the partials (fixed order) */
-__global__ void k_dots_sum(const double *partial, int nk, int nblk, double *out)
-{
-    const int k = threadIdx.x, b = blockIdx.x;
-    if (k >= nk) return;
-    const double *p = partial + ((size_t)b * nk + k) * nblk;
-    double s = 0.0;
-    for (int q = 0; q < nblk; ++q) s += p[q];
-    out[(size_t)b * nk + k] = s;
-}
-
-/* w -= sum_k h[b, k] V_k  (k < nk) */
-__global__ void k_axpy_v(const AdjVec *Vs, const int *live, int nk, const double *h, int hstride)
+/* partials of <V_k, W> for k in [blockIdx.y * DOT_KB, +DOT_KB) and k < nk,
+ * and (wnorm, blockIdx.y 0) of <W, W> from the same read of W; partial
+ * layout ((b * nk + k) * nblk + chunk), <W, W> in wpart (b * nblk + chunk) */
+__global__ void __launch_bounds__(DOT_T) k_dots(const AdjVec *Vs, const int *live, int nk,
+                                                int wnorm, double *partial, double *wpart,
+                                                int nblk)
 {
     const AdjVec &W = Vs[blockIdx.z];
     if (!live[blockIdx.z]) return;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= W.n4) return;
-    double acc = W.W[q];
-    for (int k = 0; k < nk; ++k) acc -= h[(size_t)blockIdx.z * hstride + k] * W.V[(size_t)k * W.n4 + q];
-    W.W[q] = acc;
+    __shared__ double red[4 * (DOT_KB + 1)];
+    const long base = (long)blockIdx.x * DOT_T * DOT_PER + threadIdx.x;
+    const int k0 = (int)blockIdx.y * DOT_KB;
+    double w[DOT_PER];
+#pragma unroll
+    for (int t = 0; t < DOT_PER; ++t) {
+        const long q = base + (long)t * DOT_T;
+        w[t] = q < W.n4 ? W.W[q] : 0.0;
+    }
+    double acc[DOT_KB + 1];
+#pragma unroll
+    for (int i = 0; i <= DOT_KB; ++i) acc[i] = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < DOT_KB; ++kk) {
+        if (k0 + kk < nk) {
+            const double *a = W.V + (size_t)(k0 + kk) * W.n4;
+#pragma unroll
+            for (int t = 0; t < DOT_PER; ++t) {
+                const long q = base + (long)t * DOT_T;
+                if (q < W.n4) acc[kk] += a[q] * w[t];
+            }
+        }
+    }
+    const bool wn = wnorm && blockIdx.y == 0;
+    if (wn)
+#pragma unroll
+        for (int t = 0; t < DOT_PER; ++t) acc[DOT_KB] += w[t] * w[t];
+    wg_sums<DOT_KB + 1>(acc, red);
+    const int i = threadIdx.x;
+    if (i < DOT_KB && k0 + i < nk) {
+        const double v = ((red[i] + red[(DOT_KB + 1) + i]) + red[2 * (DOT_KB + 1) + i]) +
+                         red[3 * (DOT_KB + 1) + i];
+        partial[((size_t)blockIdx.z * nk + k0 + i) * nblk + blockIdx.x] = v;
+    }
+    if (wn && i == DOT_KB) {
+        const double v = ((red[i] + red[(DOT_KB + 1) + i]) + red[2 * (DOT_KB + 1) + i]) +
+                         red[3 * (DOT_KB + 1) + i];
+        wpart[(size_t)blockIdx.z * nblk + blockIdx.x] = v;
+    }
+}
+
+/* partials of <X, X> (which 0: W, 1: G), layout (b * nblk + chunk) */
+__global__ void __launch_bounds__(DOT_T) k_norm2(const AdjVec *Vs, const int *live, int which,
+                                                 double *partial, int nblk)
+{
+    const AdjVec &W = Vs[blockIdx.z];
+    if (!live[blockIdx.z]) return;
+    __shared__ double red[4];
+    const double *x = which == 0 ? W.W : W.G;
+    const long base = (long)blockIdx.x * DOT_T * DOT_PER + threadIdx.x;
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int t = 0; t < DOT_PER; ++t) {
+        const long q = base + (long)t * DOT_T;
+        if (q < W.n4) acc[0] += x[q] * x[q];
+    }
+    wg_sums<1>(acc, red);
+    if (threadIdx.x == 0)
+        partial[(size_t)blockIdx.z * nblk + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+/* out[b * nk + k] = the sum of product k's partials of problem b (grid
+ * (nk, nb)); 0 for a problem that is not live */
+__global__ void __launch_bounds__(DOT_T) k_dots_sum(const double *partial, const int *live, int nk,
+                                                    int nblk, double *out)
+{
+    const int k = blockIdx.x, b = blockIdx.y;
+    if (!live[b]) {
+        if (threadIdx.x == 0) out[(size_t)b * nk + k] = 0.0;
+        return;
+    }
+    __shared__ double red[4];
+    const double *p = partial + ((size_t)b * nk + k) * nblk;
+    double s[1] = {0.0};
+    for (int q = threadIdx.x; q < nblk; q += DOT_T) s[0] += p[q];
+    wg_sums<1>(s, red);
+    if (threadIdx.x == 0) out[(size_t)b * nk + k] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+/* W -= sum_k h[b, k] V_k (k < nk), and the partials of <W, W> of the result
+ * (layout b * nblk + chunk) from the same pass */
+__global__ void __launch_bounds__(DOT_T) k_orth(const AdjVec *Vs, const int *live, int nk,
+                                                const double *h, int hstride, double *wpart,
+                                                int nblk)
+{
+    const AdjVec &W = Vs[blockIdx.z];
+    if (!live[blockIdx.z]) return;
+    __shared__ double red[4];
+    const double *hb = h + (size_t)blockIdx.z * hstride;
+    const long base = (long)blockIdx.x * DOT_T * DOT_PER + threadIdx.x;
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int t = 0; t < DOT_PER; ++t) {
+        const long q = base + (long)t * DOT_T;
+        if (q < W.n4) {
+            double x = W.W[q];
+            for (int k = 0; k < nk; ++k) x -= hb[k] * W.V[(size_t)k * W.n4 + q];
+            W.W[q] = x;
+            acc[0] += x * x;
+        }
+    }
+    wg_sums<1>(acc, red);
+    if (threadIdx.x == 0)
+        wpart[(size_t)blockIdx.z * nblk + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 /* dst = c[b] * src with src/dst picked by sel: 0 V_j = W / h, 1 W = G - W (residual) */
@@ -1166,7 +1254,7 @@ static int build_work(mgf_solver *s, int restart, AdjWork **out)
         mgf_adjoint_release(w);
         return mgf_set_error(s, MGF_ERR_DEVICE, "adjoint pointer upload failed");
     }
-    const size_t small = (size_t)nb * (2 * (restart + 2) + 8) + 16;
+    const size_t small = (size_t)nb * (3 * (restart + 2) + 12) + 16;
     w->d_small = dalloc(w, small);
     if (hipMalloc(&w->d_gin, nb * sizeof(double *)) != hipSuccess ||
         hipMalloc(&w->d_seeds, 2 * nb * sizeof(MetSeed)) != hipSuccess || !w->d_small) {
@@ -1285,16 +1373,47 @@ static void apply_kt(AdjWork *w, bool from_x, hipStream_t st)
                            (const double *const *)in, w->d_kout);
 }
 
-/* dots per live problem: out[b * nk + k] */
-static int dots(mgf_solver *s, AdjWork *w, int nk, int mode, double *d_out, hipStream_t st)
+static int dot_chunks(const AdjWork *w)
 {
-    const int nblk = (int)((w->maxn4 + DOT_T * DOT_PER - 1) / (DOT_T * DOT_PER));
-    int rc = ensure_partial(s, w, (size_t)w->nb * nk * nblk);
+    return (int)((w->maxn4 + DOT_T * DOT_PER - 1) / (DOT_T * DOT_PER));
+}
+
+/* d_out[b] = <X, X> per live problem (which 0: W, 1: G), 0 for the others */
+static int norms(mgf_solver *s, AdjWork *w, int which, double *d_out, hipStream_t st)
+{
+    const int nblk = dot_chunks(w), nb = w->nb;
+    int rc = ensure_partial(s, w, (size_t)nb * nblk);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_dots, dim3((unsigned)nblk, (unsigned)nk, (unsigned)w->nb), dim3(DOT_T), 0,
-                       st, w->d_vec, w->d_live, nk, mode, w->d_partial, nblk);
-    hipLaunchKernelGGL(k_dots_sum, dim3((unsigned)w->nb), dim3(256), 0, st, w->d_partial, nk,
-                       nblk, d_out);
+    hipLaunchKernelGGL(k_norm2, dim3((unsigned)nblk, 1, (unsigned)nb), dim3(DOT_T), 0, st, w->d_vec,
+                       w->d_live, which, w->d_partial, nblk);
+    hipLaunchKernelGGL(k_dots_sum, dim3(1, (unsigned)nb), dim3(DOT_T), 0, st, w->d_partial,
+                       w->d_live, 1, nblk, d_out);
+    return MGF_OK;
+}
+
+/* One classical Gram-Schmidt pass on W of every live problem against
+ * V_0..V_{nk-1}: d_hk[b * nk + k] = <V_k, W>, then W -= sum_k h_k V_k;
+ * d_wn0[b] = <W, W> before the pass (when not null, from the same read of
+ * W) and d_wn1[b] after it (from the subtraction's own pass). */
+static int cgs_pass(mgf_solver *s, AdjWork *w, int nk, double *d_hk, double *d_wn0, double *d_wn1,
+                    hipStream_t st)
+{
+    const int nblk = dot_chunks(w), nb = w->nb;
+    int rc = ensure_partial(s, w, (size_t)nb * (nk + 1) * nblk);
+    if (rc) return rc;
+    double *part = w->d_partial, *wpart = part + (size_t)nb * nk * nblk;
+    const unsigned gy = (unsigned)((nk + DOT_KB - 1) / DOT_KB);
+    hipLaunchKernelGGL(k_dots, dim3((unsigned)nblk, gy, (unsigned)nb), dim3(DOT_T), 0, st,
+                       w->d_vec, w->d_live, nk, d_wn0 != nullptr, part, wpart, nblk);
+    hipLaunchKernelGGL(k_dots_sum, dim3((unsigned)nk, (unsigned)nb), dim3(DOT_T), 0, st, part,
+                       w->d_live, nk, nblk, d_hk);
+    if (d_wn0)
+        hipLaunchKernelGGL(k_dots_sum, dim3(1, (unsigned)nb), dim3(DOT_T), 0, st, wpart, w->d_live,
+                           1, nblk, d_wn0);
+    hipLaunchKernelGGL(k_orth, dim3((unsigned)nblk, 1, (unsigned)nb), dim3(DOT_T), 0, st, w->d_vec,
+                       w->d_live, nk, d_hk, nk, wpart, nblk);
+    hipLaunchKernelGGL(k_dots_sum, dim3(1, (unsigned)nb), dim3(DOT_T), 0, st, wpart, w->d_live, 1,
+                       nblk, d_wn1);
     return MGF_OK;
 }
 
@@ -1399,16 +1518,18 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
     const int nb = w->nb, m = o.restart;
     int nswm, nswp;
     sweeps_of(s, &nswm, &nswp);
-    /* small device scratch: h (nb x (m + 2)), y (nb x m), c (nb), J (2 nb) */
+    /* small device scratch: h, h2 (nb x (m + 2) each), y (nb x m), c (nb), J (2 nb), the
+     * norms n0, n1, n2 (nb each) */
     double *d_h = w->d_small, *d_y = d_h + (size_t)nb * (m + 2), *d_c = d_y + (size_t)nb * m;
-    double *d_J = d_c + nb;
+    double *d_J = d_c + nb, *d_h2 = d_J + 2 * nb, *d_n0 = d_h2 + (size_t)nb * (m + 2);
+    double *d_n1 = d_n0 + nb, *d_n2 = d_n1 + nb;
     /* the objective's dJ/dx and J */
     for (const AdjGroup &G : w->g0s)
         hipLaunchKernelGGL(k_adj_objective, dim3((unsigned)G.count), dim3(256), 0, st, G.d,
                            w->d_vec, objective, d_J);
     std::vector<double> hJ(2 * nb), gnorm(nb), beta(nb);
     AHIP(s, hipMemcpyAsync(hJ.data(), d_J, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
-    if ((rc = dots(s, w, 1, 2, d_h, st))) return rc;
+    if ((rc = norms(s, w, 1, d_h, st))) return rc;
     std::vector<double> hh((size_t)nb * (m + 2));
     AHIP(s, hipMemcpyAsync(hh.data(), d_h, nb * sizeof(double), hipMemcpyDeviceToHost, st));
     AHIP(s, hipStreamSynchronize(st));
@@ -1449,7 +1570,7 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
         /* W = G - K^T X, beta = ||W||, V_0 = W / beta */
         apply_kt(w, true, st);
         vec_launch(w, 1, 0, d_c, st);
-        if ((rc = dots(s, w, 1, 1, d_h, st))) return rc;
+        if ((rc = norms(s, w, 0, d_h, st))) return rc;
         AHIP(s, hipMemcpyAsync(hh.data(), d_h, nb * sizeof(double), hipMemcpyDeviceToHost, st));
         AHIP(s, hipStreamSynchronize(st));
         for (int b = 0; b < nb; ++b) {
@@ -1485,28 +1606,49 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
             set_f0_from(w, 0, j, 0, nullptr, 0, st);
             vcycle(w, nswm, nswp, st);
             apply_kt(w, false, st);
-            /* classical Gram-Schmidt, twice */
-            std::vector<double> hsum((size_t)nb * (j + 1), 0.0);
-            for (int pass = 0; pass < 2; ++pass) {
-                if ((rc = dots(s, w, j + 1, 0, d_h, st))) return rc;
-                {
-                    const dim3 g((unsigned)((w->maxn4 + 255) / 256), 1, (unsigned)nb);
-                    hipLaunchKernelGGL(k_axpy_v, g, dim3(256), 0, st, w->d_vec, w->d_live, j + 1,
-                                       d_h, j + 1);
-                }
-                AHIP(s, hipMemcpyAsync(hh.data(), d_h, (size_t)nb * (j + 1) * sizeof(double),
-                                       hipMemcpyDeviceToHost, st));
-                AHIP(s, hipStreamSynchronize(st));
-                for (size_t q = 0; q < (size_t)nb * (j + 1); ++q) hsum[q] += hh[q];
-            }
-            if ((rc = dots(s, w, 1, 1, d_h, st))) return rc;
-            AHIP(s, hipMemcpyAsync(hh.data(), d_h, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            /* classical Gram-Schmidt, and a second pass for the problems whose W
+             * kept less than 1/sqrt(2) of its norm through the first (the
+             * reorthogonalisation criterion of Daniel, Gragg, Kaufman and
+             * Stewart, 1976): after it W is orthogonal to the V_k to working
+             * precision, and most steps need one pass */
+            const size_t nh = (size_t)nb * (j + 1);
+            std::vector<double> hsum(nh), n0(nb), n1(nb);
+            if ((rc = cgs_pass(s, w, j + 1, d_h, d_n0, d_n1, st))) return rc;
+            AHIP(s, hipMemcpyAsync(hsum.data(), d_h, nh * sizeof(double), hipMemcpyDeviceToHost,
+                                   st));
+            AHIP(s, hipMemcpyAsync(n0.data(), d_n0, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+            AHIP(s, hipMemcpyAsync(n1.data(), d_n1, nb * sizeof(double), hipMemcpyDeviceToHost, st));
             AHIP(s, hipStreamSynchronize(st));
+            {
+                const std::vector<int> live_step = w->h_live;
+                bool again = false;
+                for (int b = 0; b < nb; ++b) {
+                    w->h_live[b] = live_step[b] && !(n1[b] >= 0.5 * n0[b]);
+                    again |= w->h_live[b] != 0;
+                }
+                if (again) {
+                    std::vector<double> h2(nh), n2(nb);
+                    if ((rc = upload_live(s, w, st))) return rc;
+                    if ((rc = cgs_pass(s, w, j + 1, d_h2, nullptr, d_n2, st))) return rc;
+                    AHIP(s, hipMemcpyAsync(h2.data(), d_h2, nh * sizeof(double),
+                                           hipMemcpyDeviceToHost, st));
+                    AHIP(s, hipMemcpyAsync(n2.data(), d_n2, nb * sizeof(double),
+                                           hipMemcpyDeviceToHost, st));
+                    AHIP(s, hipStreamSynchronize(st));
+                    for (int b = 0; b < nb; ++b) {
+                        if (!w->h_live[b]) continue;
+                        for (int k = 0; k <= j; ++k) hsum[(size_t)b * (j + 1) + k] += h2[(size_t)b * (j + 1) + k];
+                        n1[b] = n2[b];
+                    }
+                }
+                w->h_live = live_step;
+                if (again && (rc = upload_live(s, w, st))) return rc;
+            }
             for (int b = 0; b < nb; ++b) {
                 hc[b] = 0.0;
                 if (!w->h_live[b]) continue;
                 double *Hb = &H[(size_t)b * (m + 1) * m];
-                const double hn = sqrt(hh[b]);
+                const double hn = sqrt(n1[b]);
                 for (int k = 0; k <= j; ++k) Hb[k * m + j] = hsum[(size_t)b * (j + 1) + k];
                 Hb[(j + 1) * m + j] = hn;
                 /* apply the previous rotations, then a new one */
@@ -1575,7 +1717,7 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
     if ((rc = upload_live(s, w, st))) return rc;
     apply_kt(w, true, st);
     vec_launch(w, 1, 0, d_c, st);
-    if ((rc = dots(s, w, 1, 1, d_h, st))) return rc;
+    if ((rc = norms(s, w, 0, d_h, st))) return rc;
     AHIP(s, hipMemcpyAsync(hh.data(), d_h, nb * sizeof(double), hipMemcpyDeviceToHost, st));
     AHIP(s, hipStreamSynchronize(st));
     int unconverged = 0;
