@@ -48,6 +48,9 @@ typedef struct mgf_adjoint_opts {
     int only_converged;/* 1 (default): problems whose last solve did not meet its
                           tolerance are skipped (gradients NaN, iters -1): the
                           adjoint is that of the steady solution */
+    int single_colour; /* 1: the preconditioner's red-black sweeps as one launch per
+                          colour and sweep instead of several sweeps per launch
+                          (default 0); the same result bit for bit, for checking */
 } mgf_adjoint_opts;
 
 void mgf_adjoint_opts_default(mgf_adjoint_opts *o);

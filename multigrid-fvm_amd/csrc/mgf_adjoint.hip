@@ -79,6 +79,7 @@ struct AdjLevel {
     const int *live;
     double *e[4], *f[4], *r[4];
     double *dp, *apu, *apv;
+    double *tmp;        /* the other buffer of the blocked sweeps */
 };
 
 struct AdjGroup {
@@ -553,6 +554,116 @@ __global__ void k_adj_rb_p(const AdjLevel *As, int colour)
     A.dp[ij] = (s + A.r[2][ij]) / ap;
 }
 
+/* S red-black sweeps of one multiplier (FIELD 0 u, 1 v, 2 the continuity
+ * correction dp, 3 sc) in one launch, temporally blocked: a workgroup takes
+ * an ATB_X x ATB_Y tile and a halo of H = 2S points into LDS, forms each
+ * updatable point's transposed coefficients, right-hand side and diagonal
+ * once, and colour step m (colour (m - 1) & 1, as the single-colour
+ * launches run them) updates the points at distance >= m from the region's
+ * edge; after 2S steps the tile holds what S sweeps of k_adj_rb_s /
+ * k_adj_rb_uv / k_adj_rb_p give, bit for bit (same expressions).  Points
+ * outside the level's arrays hold 0 and are only read by points that are
+ * never updated.  The halo is other workgroups' tiles, so the launch reads
+ * one buffer and writes the other (dir 0: the multiplier's array to tmp,
+ * 1: back), every point of its tile, updated or not: the launches of a
+ * relaxation come in pairs, and the second leaves the result in place. */
+#define ATB_X 64
+#define ATB_Y 16
+#define ATB_NT 512
+template <int FIELD, int S>
+__global__ void __launch_bounds__(ATB_NT) k_adj_rbT(const AdjLevel *As, int dir)
+{
+    constexpr int H = 2 * S, RW = ATB_X + 2 * H, RH = ATB_Y + 2 * H, NR = RW * RH;
+    constexpr int NPT = (NR + ATB_NT - 1) / ATB_NT;
+    const AdjLevel &A = As[blockIdx.z];
+    if (!A.live[A.inst]) return;
+    const DevLevel &L = A.P;
+    const int nx = L.npx, ny = L.npy;
+    __shared__ double t[NR];
+    const int ri0 = (int)blockIdx.x * ATB_X - H, rj0 = (int)blockIdx.y * ATB_Y - H;
+    const int tid = (int)(threadIdx.y * ATB_X + threadIdx.x);
+    double *const xf = FIELD == 2 ? A.dp : A.e[FIELD];
+    const double *src = dir ? A.tmp : xf;
+    double *dst = dir ? xf : A.tmp;
+    const double *rhs = FIELD == 3 ? A.f[3] : A.r[FIELD];
+    double cw[NPT], ce[NPT], cs[NPT], cn[NPT], rr[NPT], dg[NPT];
+    unsigned meta[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int p = tid + k * ATB_NT;
+        double val = 0.0;
+        unsigned m = 0;
+        cw[k] = ce[k] = cs[k] = cn[k] = rr[k] = 0.0;
+        dg[k] = 1.0;
+        if (p < NR) {
+            const int r = p / RW, c = p - r * RW;
+            const int gi = ri0 + c, gj = rj0 + r;
+            if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+                const int g = gj * nx + gi;
+                val = src[g];
+                bool upd;
+                if (FIELD == 0) upd = urow(L, gi, gj);
+                else if (FIELD == 1) upd = vrow(L, gi, gj);
+                else if (FIELD == 3) upd = srow(L, gi, gj);
+                else
+                    upd = crow(L, gi, gj) &&
+                          !(A.apu[g - 1] >= ADJ_FIXED_FACE && A.apu[g] >= ADJ_FIXED_FACE &&
+                            A.apv[g - nx] >= ADJ_FIXED_FACE && A.apv[g] >= ADJ_FIXED_FACE);
+                if (upd) {
+                    if (FIELD == 2) {
+                        cw[k] = L.axax / A.apu[g - 1];
+                        ce[k] = L.axax / A.apu[g];
+                        cs[k] = L.ayay / A.apv[g - nx];
+                        cn[k] = L.ayay / A.apv[g];
+                        dg[k] = ((cw[k] + ce[k]) + cs[k]) + cn[k];
+                    } else {
+                        double ap;
+                        const Tco tc = FIELD == 0 ? tco_u(L, gi, gj, nullptr)
+                                                  : (FIELD == 1 ? tco_v(L, gi, gj, nullptr)
+                                                                : tco_s(L, gi, gj, &ap));
+                        cw[k] = tc.w;
+                        ce[k] = tc.e;
+                        cs[k] = tc.s;
+                        cn[k] = tc.n;
+                        dg[k] = FIELD == 0 ? A.apu[g] : (FIELD == 1 ? A.apv[g] : ap);
+                    }
+                    rr[k] = rhs[g];
+                    m = (unsigned)((gi + gj) & 1) << 8 | (1u << 9);
+                }
+                /* bit 10: inside the level's arrays, written back */
+                m |= (unsigned)min(min(c, RW - 1 - c), min(r, RH - 1 - r)) | (1u << 10);
+            }
+            t[p] = val;
+        }
+        meta[k] = m;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int step = 1; step <= 2 * S; ++step) {
+        const unsigned want = (1u << 9) | ((unsigned)((step - 1) & 1) << 8);
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const unsigned m = meta[k];
+            if ((m & 0x300u) == want && (int)(m & 0xffu) >= step) {
+                const int p = tid + k * ATB_NT;
+                const double sum = cw[k] * t[p - 1] + ce[k] * t[p + 1] + cs[k] * t[p - RW] +
+                                   cn[k] * t[p + RW];
+                t[p] = (sum + rr[k]) / dg[k];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const unsigned m = meta[k];
+        if ((m & (1u << 10)) && (int)(m & 0xffu) >= H) {
+            const int p = tid + k * ATB_NT;
+            const int r = p / RW, c = p - r * RW;
+            dst[(rj0 + r) * nx + (ri0 + c)] = t[p];
+        }
+    }
+}
+
 /* the update: e_u -= areax (dp_P - dp_E) / apu, e_v likewise, e_c += relxp dp */
 __global__ void k_adj_update(const AdjLevel *As)
 {
@@ -1010,6 +1121,7 @@ void **mgf_adjoint_slot(mgf_solver *s);
 
 struct AdjWork {
     int nb = 0, restart = 0;
+    int single_colour = 0;                          /* sweeps as single-colour launches */
     std::vector<int> ng;
     std::vector<std::vector<AdjLevel>> lv;          /* host copies per problem and level */
     std::vector<AdjVec> vec;
@@ -1086,12 +1198,12 @@ static int build_work(mgf_solver *s, int restart, AdjWork **out)
             A.P = lv[ig];
             A.inst = b;
             const size_t n = (size_t)lv[ig].npx * lv[ig].npy;
-            double *blk = dalloc(w, 15 * n);
+            double *blk = dalloc(w, 16 * n);
             if (!blk) {
                 mgf_adjoint_release(w);
                 return mgf_set_error(s, MGF_ERR_DEVICE, "adjoint workspace allocation failed");
             }
-            if (hipMemsetAsync(blk, 0, 15 * n * sizeof(double), st) != hipSuccess) {
+            if (hipMemsetAsync(blk, 0, 16 * n * sizeof(double), st) != hipSuccess) {
                 mgf_adjoint_release(w);
                 return mgf_set_error(s, MGF_ERR_DEVICE, "adjoint workspace clear failed");
             }
@@ -1103,6 +1215,7 @@ static int build_work(mgf_solver *s, int restart, AdjWork **out)
             A.dp = blk + 12 * n;
             A.apu = blk + 13 * n;
             A.apv = blk + 14 * n;
+            A.tmp = blk + 15 * n;
         }
         const long n4 = 4L * lv[0].npx * lv[0].npy;
         w->maxn4 = std::max<long>(w->maxn4, n4);
@@ -1293,27 +1406,44 @@ static dim3 agrid(int nx, int ny, int nz)
 static const dim3 ABLK(ABX, ABY);
 
 /* one adjoint SIMPLE relaxation of a group */
-static void relax_group(const AdjGroup &G, int nswm, int nswp, hipStream_t st)
+/* nsw red-black sweeps of one multiplier: an even number of launches of up
+ * to three sweeps each (k_adj_rbT, to tmp and back), or single colours in
+ * place (one sweep, or single) */
+template <int FIELD>
+static void adj_sweeps(const AdjGroup &G, int nsw, bool single, hipStream_t st)
 {
     const dim3 g = agrid(G.npx, G.npy, G.count);
-    for (int k = 0; k < nswm; ++k) {
-        hipLaunchKernelGGL(k_adj_rb_s, g, ABLK, 0, st, G.d, 0);
-        hipLaunchKernelGGL(k_adj_rb_s, g, ABLK, 0, st, G.d, 1);
+    if (single || nsw < 2) {
+        for (int k = 0; k < nsw; ++k)
+            for (int colour = 0; colour < 2; ++colour) {
+                if (FIELD == 3) hipLaunchKernelGGL(k_adj_rb_s, g, ABLK, 0, st, G.d, colour);
+                else if (FIELD == 2) hipLaunchKernelGGL(k_adj_rb_p, g, ABLK, 0, st, G.d, colour);
+                else hipLaunchKernelGGL(k_adj_rb_uv, g, ABLK, 0, st, G.d, FIELD, colour);
+            }
+        return;
     }
+    const dim3 gt((unsigned)((G.npx + ATB_X - 1) / ATB_X), (unsigned)((G.npy + ATB_Y - 1) / ATB_Y),
+                  (unsigned)G.count);
+    const dim3 bt(ATB_X, ATB_NT / ATB_X);
+    const int nl = 2 * ((nsw + 5) / 6);
+    for (int k = 0; k < nl; ++k) {
+        const int n = nsw / nl + (k < nsw % nl ? 1 : 0);
+        const int dir = k & 1;
+        if (n == 3) hipLaunchKernelGGL((k_adj_rbT<FIELD, 3>), gt, bt, 0, st, G.d, dir);
+        else if (n == 2) hipLaunchKernelGGL((k_adj_rbT<FIELD, 2>), gt, bt, 0, st, G.d, dir);
+        else hipLaunchKernelGGL((k_adj_rbT<FIELD, 1>), gt, bt, 0, st, G.d, dir);
+    }
+}
+
+static void relax_group(const AdjGroup &G, int nswm, int nswp, bool single, hipStream_t st)
+{
+    const dim3 g = agrid(G.npx, G.npy, G.count);
+    adj_sweeps<3>(G, nswm, single, st);
     hipLaunchKernelGGL(k_adj_rhs_uv, g, ABLK, 0, st, G.d);
-    for (int k = 0; k < nswm; ++k) {
-        hipLaunchKernelGGL(k_adj_rb_uv, g, ABLK, 0, st, G.d, 0, 0);
-        hipLaunchKernelGGL(k_adj_rb_uv, g, ABLK, 0, st, G.d, 0, 1);
-    }
-    for (int k = 0; k < nswm; ++k) {
-        hipLaunchKernelGGL(k_adj_rb_uv, g, ABLK, 0, st, G.d, 1, 0);
-        hipLaunchKernelGGL(k_adj_rb_uv, g, ABLK, 0, st, G.d, 1, 1);
-    }
+    adj_sweeps<0>(G, nswm, single, st);
+    adj_sweeps<1>(G, nswm, single, st);
     hipLaunchKernelGGL(k_adj_defect, g, ABLK, 0, st, G.d);
-    for (int k = 0; k < nswp; ++k) {
-        hipLaunchKernelGGL(k_adj_rb_p, g, ABLK, 0, st, G.d, 0);
-        hipLaunchKernelGGL(k_adj_rb_p, g, ABLK, 0, st, G.d, 1);
-    }
+    adj_sweeps<2>(G, nswp, single, st);
     hipLaunchKernelGGL(k_adj_update, g, ABLK, 0, st, G.d);
 }
 
@@ -1333,7 +1463,7 @@ static void vcycle(AdjWork *w, int nswm, int nswp, hipStream_t st)
         for (int q : w->depth_groups[d]) {
             const AdjGroup &G = w->groups[q];
             const int reps = d == 0 ? ADJ_COARSE_RELAX : 1;
-            for (int r = 0; r < reps; ++r) relax_group(G, nswm, nswp, st);
+            for (int r = 0; r < reps; ++r) relax_group(G, nswm, nswp, w->single_colour, st);
             if (d >= 1) hipLaunchKernelGGL(k_adj_resid, agrid(G.npx, G.npy, G.count), ABLK, 0, st, G.d);
         }
         if (d >= 1)
@@ -1348,7 +1478,8 @@ static void vcycle(AdjWork *w, int nswm, int nswp, hipStream_t st)
             const AdjPair &R = w->pairs[p];
             hipLaunchKernelGGL(k_adj_prolong, agrid(R.npxf, R.npyf, R.count), ABLK, 0, st, R.f, R.c);
         }
-        for (int q : w->depth_groups[d]) relax_group(w->groups[q], nswm, nswp, st);
+        for (int q : w->depth_groups[d])
+            relax_group(w->groups[q], nswm, nswp, w->single_colour, st);
     }
 }
 
@@ -1484,6 +1615,7 @@ extern "C" void mgf_adjoint_opts_default(mgf_adjoint_opts *o)
     o->max_iter = 600;
     o->restart = 40;
     o->only_converged = 1;
+    o->single_colour = 0;
 }
 
 /* Givens rotation of (a, b) */
@@ -1514,6 +1646,7 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
     AdjWork *w = nullptr;
     int rc = prepare(s, o.restart, &w);
     if (rc) return rc;
+    w->single_colour = o.single_colour != 0;
     hipStream_t st = (hipStream_t)mgf_stream(s);
     const int nb = w->nb, m = o.restart;
     int nswm, nswp;
@@ -1873,6 +2006,7 @@ extern "C" int mgf_adjoint_apply(mgf_solver *s, int b, int op, const double *in,
     AdjWork *w = (AdjWork *)*mgf_adjoint_slot(s);
     int rc = prepare(s, w ? w->restart : o.restart, &w);
     if (rc) return rc;
+    w->single_colour = 0;
     hipStream_t st = (hipStream_t)mgf_stream(s);
     for (int q = 0; q < w->nb; ++q) w->h_live[q] = q == b;
     if ((rc = upload_live(s, w, st))) return rc;

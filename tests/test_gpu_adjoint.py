@@ -263,6 +263,35 @@ def test_batched_adjoint_workload_reduced(tmp_path):
         _close(resm[q][2], res[k][2], 1e-5, f"mixed amuf {k}")
 
 
+def test_blocked_sweeps_equal_single_colour(tmp_path):
+    """The preconditioner's sweeps run several per launch on LDS tiles (two
+    launches per field, to a scratch array and back); the same sweeps as
+    one launch per colour give the same bits: every gradient, J, iteration
+    count and final residual, for both objectives of the 256^2
+    differentiation workload and for a batch of two 512^2 problems."""
+    import mgfvm
+    from mgfvm import adjoint as gadj
+    from mgfvm.inputs import adjoint_batch, sensitivity_256
+    p, inlet = sensitivity_256(1)
+    pin, ppat = p.write(str(tmp_path / "g256"))
+    probs = [q.replace(convergence=1e-10) for q in adjoint_batch(2)]
+    pairs = [q.write(str(tmp_path / f"b{k}")) for k, q in enumerate(probs)]
+    _say("[gpu] blocked vs single-colour adjoint sweeps: 256^2 and 2 x 512^2")
+    with mgfvm.Solver(pin, ppat, inlet_u=np.array(inlet)) as s, mgfvm.Batch(pairs) as bt:
+        assert s.solve()[2]
+        bt.solve()
+        for solver, objs in ((s, (0, 1)), (bt, (0,))):
+            for obj in objs:
+                a = gadj.batch_adjoint(solver, obj, gadj.opts(), allow_unconverged=True)
+                b = gadj.batch_adjoint(solver, obj, gadj.opts(single_colour=True),
+                                       allow_unconverged=True)
+                for ra, rb in zip(a, b):
+                    assert ra[4] == rb[4]
+                    va = np.concatenate([[ra[0], ra[2], ra[3], ra[5]], ra[1]])
+                    vb = np.concatenate([[rb[0], rb[2], rb[3], rb[5]], rb[1]])
+                    assert np.array_equal(va.view(np.uint64), vb.view(np.uint64)), obj
+
+
 def test_unconverged_solver_is_refused(tmp_path):
     import mgfvm
     from mgfvm import _lib
