@@ -1052,15 +1052,30 @@ __global__ void __launch_bounds__(DOT_T) k_orth(const AdjVec *Vs, const int *liv
     __shared__ double red[4];
     const double *hb = h + (size_t)blockIdx.z * hstride;
     const long base = (long)blockIdx.x * DOT_T * DOT_PER + threadIdx.x;
+    /* the thread's entries in registers, one coefficient and one load per
+     * entry for each Krylov vector (each entry's subtractions in k order) */
+    double x[DOT_PER];
+#pragma unroll
+    for (int t = 0; t < DOT_PER; ++t) {
+        const long q = base + (long)t * DOT_T;
+        x[t] = q < W.n4 ? W.W[q] : 0.0;
+    }
+    for (int k = 0; k < nk; ++k) {
+        const double hk = hb[k];
+        const double *v = W.V + (size_t)k * W.n4;
+#pragma unroll
+        for (int t = 0; t < DOT_PER; ++t) {
+            const long q = base + (long)t * DOT_T;
+            if (q < W.n4) x[t] -= hk * v[q];
+        }
+    }
     double acc[1] = {0.0};
 #pragma unroll
     for (int t = 0; t < DOT_PER; ++t) {
         const long q = base + (long)t * DOT_T;
         if (q < W.n4) {
-            double x = W.W[q];
-            for (int k = 0; k < nk; ++k) x -= hb[k] * W.V[(size_t)k * W.n4 + q];
-            W.W[q] = x;
-            acc[0] += x * x;
+            W.W[q] = x[t];
+            acc[0] += x[t] * x[t];
         }
     }
     wg_sums<1>(acc, red);
