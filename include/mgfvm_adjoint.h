@@ -27,7 +27,11 @@
  * The reference has no derivative capability; these calls replace none of
  * its routines.  Penalised unknowns (faces and cells of obstacles, 1e30 on
  * their diagonal) are held fixed, which moves the gradient by O(1e-30)
- * relative.
+ * relative.  Every ghost value is held fixed too, which is exact for
+ * Dirichlet ghosts (ibc 1) and discrete inlets and outlets; a solver with
+ * any other boundary type (p' Dirichlet rows, periodic, outflow) is refused
+ * with MGF_ERR_CONFIG by mgf_adjoint, mgf_batch_adjoint and
+ * mgf_adjoint_apply.
  */
 #ifndef MGFVM_ADJOINT_H
 #define MGFVM_ADJOINT_H

@@ -1584,8 +1584,33 @@ static int sweeps_of(mgf_solver *s, int *nswm, int *nswp)
     return MGF_OK;
 }
 
+/* The residual rows treat every ghost value as fixed, which is the steady
+ * problem with Dirichlet ghosts (ibc 1) on every side; discrete inlets and
+ * outlets set fixed ghosts too.  Periodic (3) and outflow (4) ghosts are
+ * copies of unknowns, and p' Dirichlet rows (2) leave their cells' mass
+ * balance unenforced: the converged state is then not a zero of these rows,
+ * so such problems are refused. */
+static int check_boundaries(mgf_solver *s)
+{
+    const int n = mgf_batch_size(s);
+    for (int b = 0; b < n; ++b) {
+        const mgf_problem *p = mgf_instance_problem(s, b);
+        for (int k = 0; k < 4; ++k)
+            if (p->ibc[k] != 1) {
+                char msg[160];
+                snprintf(msg, sizeof msg,
+                         "problem %d: the adjoint needs ibc 1 on every side (side %d has %d)", b,
+                         k, p->ibc[k]);
+                return mgf_set_error(s, MGF_ERR_CONFIG, msg);
+            }
+    }
+    return MGF_OK;
+}
+
 static int prepare(mgf_solver *s, int restart, AdjWork **out)
 {
+    int brc = check_boundaries(s);
+    if (brc) return brc;
     void **slot = mgf_adjoint_slot(s);
     AdjWork *w = (AdjWork *)*slot;
     if (w && w->restart != restart) {

@@ -307,6 +307,26 @@ def test_unconverged_solver_is_refused(tmp_path):
             gadj.adjoint(s, 2)
 
 
+def test_unmodelled_boundaries_are_refused(tmp_path):
+    """The adjoint's rows hold every ghost value fixed (ibc 1): a problem
+    with a p' Dirichlet row, periodic or outflow side is refused with
+    MGF_ERR_CONFIG rather than given a wrong gradient; its objectives can
+    still be read."""
+    import mgfvm
+    from mgfvm import _lib
+    from mgfvm import adjoint as gadj
+    prob, prof = _case32()
+    for k, ibc in enumerate(((2, 2, 2, 2), (3, 1, 3, 1), (1, 4, 1, 1))):
+        pin, ppat = prob.replace(ibc=ibc, nvcycle=40, convergence=0.0).write(
+            str(tmp_path / f"b{k}"))
+        with mgfvm.Solver(pin, ppat, inlet_u=prof) as s:
+            s.solve()
+            gadj.objectives(s)
+            with pytest.raises(_lib.MgfError) as e:
+                gadj.batch_adjoint(s, 0, gadj.opts(only_converged=False))
+            assert e.value.code == -4 and "ibc 1" in str(e.value), ibc
+
+
 def test_torch_function_backward_is_the_adjoint(tmp_path):
     """torch.autograd.Function: backward returns the adjoint gradient for
     tensor inputs (on the inputs' device and dtype), None for float
