@@ -301,11 +301,17 @@ def roofline_only(args):
         shutil.rmtree(work, ignore_errors=True)
 
 
+# GMRES restart of the batched-adjoint figure: 100 Krylov vectors (plus 4
+# work vectors) of 4 x 514^2 doubles per 512^2 problem, 0.88 GB each
+ADJ_RESTART = 100
+
+
 def adjoint_figure(count, work):
     """The batched adjoint: `count` 512^2 problems (random masks, Re in
     [20, 300]) solved in mixed precision to FP64 sum|div| < 1e-8, then the
     reverse-mode dJ1/d(inlet profile, amuf) of every converged one in one
-    batched solve; both timed after an untimed set-up run."""
+    batched solve (GMRES restarted after ADJ_RESTART vectors); both timed
+    after an untimed set-up run."""
     import mgfvm
     from mgfvm import adjoint as gadj
     from mgfvm.inputs import adjoint_batch
@@ -313,7 +319,8 @@ def adjoint_figure(count, work):
     apairs = [p.write(os.path.join(work, f"a{k}")) for k, p in enumerate(aprobs)]
     ba = mgfvm.Batch(apairs, precision=mgfvm.PREC_MIXED, device=0)
     ba.solve()
-    gadj.batch_adjoint(ba, 0, allow_unconverged=True)      # set-up, untimed
+    aopt = gadj.opts(restart=ADJ_RESTART)
+    gadj.batch_adjoint(ba, 0, aopt, allow_unconverged=True)      # set-up, untimed
     ba.reset()
     _sync(ba)
     t0 = time.perf_counter()
@@ -321,7 +328,7 @@ def adjoint_figure(count, work):
     _sync(ba)
     t_ap = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res = gadj.batch_adjoint(ba, 0, allow_unconverged=True)
+    res = gadj.batch_adjoint(ba, 0, aopt, allow_unconverged=True)
     t_aa = time.perf_counter() - t0
     ba.close()
     its = [r[4] for r in res if r[4] >= 0]
@@ -333,7 +340,9 @@ def adjoint_figure(count, work):
                     "16x16 base with porosity U[0.7, 0.95], Re U[20, 300], seeds 4000+k; "
                     "mixed-precision primal to FP64 sum|div| < 1e-8, then dJ1/d(inlet "
                     "profile, amuf) of every converged problem in one batched reverse "
-                    "solve (FP64, GMRES to 1e-11 relative residual)",
+                    f"solve (FP64, GMRES({ADJ_RESTART}) to 1e-11 relative residual)",
+        "gmres_restart": ADJ_RESTART,
+        "krylov_workspace_gb_per_problem": round((ADJ_RESTART + 4) * 4 * 514 * 514 * 8 / 1e9, 2),
         "primal_s": round(t_ap, 3), "primal_converged": int(sum(acv)),
         "adjoint_s": round(t_aa, 3), "adjoint_converged": ok,
         "adjoint_over_primal": round(t_aa / t_ap, 3),

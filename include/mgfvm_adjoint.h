@@ -48,7 +48,8 @@ extern "C" {
 typedef struct mgf_adjoint_opts {
     double rtol;       /* stop when ||dJ/dx - K^T lambda|| <= rtol ||dJ/dx|| (default 1e-11) */
     int max_iter;      /* GMRES iterations in all (default 600) */
-    int restart;       /* Krylov vectors kept before a restart (default 40) */
+    int restart;       /* Krylov vectors kept before a restart (default 60; the
+                          workspace is restart + 4 level-0 vectors per problem) */
     int only_converged;/* 1 (default): problems whose last solve did not meet its
                           tolerance are skipped (gradients NaN, iters -1): the
                           adjoint is that of the steady solution */

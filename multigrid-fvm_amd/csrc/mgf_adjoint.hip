@@ -1653,7 +1653,7 @@ extern "C" void mgf_adjoint_opts_default(mgf_adjoint_opts *o)
     if (!o) return;
     o->rtol = 1e-11;
     o->max_iter = 600;
-    o->restart = 40;
+    o->restart = 60;
     o->only_converged = 1;
     o->single_colour = 0;
 }
