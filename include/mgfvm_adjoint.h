@@ -17,7 +17,7 @@
  * the converged state takes them), applied in transposed form by a kernel
  * that recomputes every coefficient and its derivatives from the stored
  * converged state.  The solve is GMRES on the finest level, preconditioned
- * by one V-cycle of the same multigrid hierarchy: on every level an adjoint
+ * by two V-cycles of the same multigrid hierarchy: on every level an adjoint
  * SIMPLE relaxation (red-black sweeps of the transposed scalar and momentum
  * operators, and the pressure-correction equation for the adjoint
  * continuity multiplier), the reference's restriction weights for the
@@ -96,8 +96,9 @@ int mgf_adjoint_get_state(mgf_solver *s, int b, double *lam_u, double *lam_v, do
 /* Verification: one stage of the solve applied to problem b alone, at
  * the current state, on caller vectors of 4 * npx * npy doubles (level 0:
  * the u, v, continuity and scalar parts, each npx*npy).  op 0: out =
- * K^T in (the exact transposed Jacobian); op 1: out = one preconditioner
- * V-cycle applied to the right-hand side in. */
+ * K^T in (the exact transposed Jacobian); op 1: out = the
+ * preconditioner (two cycles of the adjoint V-cycle iteration from zero)
+ * applied to the right-hand side in. */
 int mgf_adjoint_apply(mgf_solver *s, int b, int op, const double *in, double *out);
 
 #ifdef __cplusplus

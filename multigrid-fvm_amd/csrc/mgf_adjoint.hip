@@ -1465,15 +1465,17 @@ static void relax_group(const AdjGroup &G, int nswm, int nswp, bool single, hipS
 #define ADJ_COARSE_RELAX 4
 
 /* e0 = M f0: one V-cycle of every live problem (level-0 f already set) */
-static void vcycle(AdjWork *w, int nswm, int nswp, hipStream_t st)
+static void vcycle(AdjWork *w, int nswm, int nswp, hipStream_t st, bool first = true)
 {
     const int D = w->D;
     /* level-0 (and every finest) correction starts at zero: restriction
-     * zeroes the coarse ones; the finest of each problem is cleared here */
-    for (const AdjGroup &G : w->g0s) {
-        const dim3 g = agrid(G.npx, G.npy, G.count);
-        hipLaunchKernelGGL(k_adj_zero_e, g, ABLK, 0, st, G.d);
-    }
+     * zeroes the coarse ones; the finest of each problem is cleared here,
+     * unless this is a further cycle of the same iteration */
+    if (first)
+        for (const AdjGroup &G : w->g0s) {
+            const dim3 g = agrid(G.npx, G.npy, G.count);
+            hipLaunchKernelGGL(k_adj_zero_e, g, ABLK, 0, st, G.d);
+        }
     for (int d = D; d >= 0; --d) {
         for (int q : w->depth_groups[d]) {
             const AdjGroup &G = w->groups[q];
@@ -1496,6 +1498,19 @@ static void vcycle(AdjWork *w, int nswm, int nswp, hipStream_t st)
         for (int q : w->depth_groups[d])
             relax_group(w->groups[q], nswm, nswp, w->single_colour, st);
     }
+}
+
+/* the preconditioner: ADJ_PRECOND_CYCLES cycles of the V-cycle iteration
+ * from e = 0 (a fixed linear operator on f).  On the 64 x 512^2 batch, two
+ * cycles take 35 GMRES iterations where one takes 50, and the whole adjoint
+ * 8% less time (the Gram-Schmidt cost grows with the iterations squared);
+ * three take 30 and more time. */
+#ifndef ADJ_PRECOND_CYCLES
+#define ADJ_PRECOND_CYCLES 2
+#endif
+static void precondition(AdjWork *w, int nswm, int nswp, hipStream_t st)
+{
+    for (int c = 0; c < ADJ_PRECOND_CYCLES; ++c) vcycle(w, nswm, nswp, st, c == 0);
 }
 
 /* ---- pieces of the solve ---------------------------------------------- */
@@ -1777,7 +1792,7 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
             if ((rc = upload_live(s, w, st))) return rc;
             /* W = K^T M V_j */
             set_f0_from(w, 0, j, 0, nullptr, 0, st);
-            vcycle(w, nswm, nswp, st);
+            precondition(w, nswm, nswp, st);
             apply_kt(w, false, st);
             /* classical Gram-Schmidt, and a second pass for the problems whose W
              * kept less than 1/sqrt(2) of its norm through the first (the
@@ -1877,7 +1892,7 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
         AHIP(s, hipMemcpyAsync(d_y, yv.data(), (size_t)nb * m * sizeof(double),
                                hipMemcpyHostToDevice, st));
         set_f0_from(w, 1, 0, m, d_y, m, st);
-        vcycle(w, nswm, nswp, st);
+        precondition(w, nswm, nswp, st);
         set_f0_from(w, 2, 0, 0, nullptr, 0, st);
         vec_launch(w, 2, 0, d_c, st);
         AHIP(s, hipStreamSynchronize(st));
@@ -2059,7 +2074,7 @@ extern "C" int mgf_adjoint_apply(mgf_solver *s, int b, int op, const double *in,
     } else {
         AHIP(s, hipMemcpyAsync(w->vec[b].V, in, n4 * sizeof(double), hipMemcpyHostToDevice, st));
         set_f0_from(w, 0, 0, 0, nullptr, 0, st);
-        vcycle(w, nswm, nswp, st);
+        precondition(w, nswm, nswp, st);
         set_f0_from(w, 2, 0, 0, nullptr, 0, st);
     }
     AHIP(s, hipMemcpyAsync(out, w->vec[b].W, n4 * sizeof(double), hipMemcpyDeviceToHost, st));
