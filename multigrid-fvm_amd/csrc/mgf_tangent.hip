@@ -3,8 +3,8 @@
  * libmgfvm_tangent.so (the FP64 path compiled with real_t = mgf_dual).
  *
  * The sensitivity library runs the FP64 path only: mgf_batch_create rejects
- * MGF_PREC_MIXED there, so the mixed-precision launchers the driver names
- * are never reached; the definitions at the end of this file stand in for
+ * MGF_PREC_MIXED there, so the mixed-precision launchers that the V-cycle
+ * code in mgf_solver.hip names are never reached; the definitions at the end of this file stand in for
  * them and report the misuse if one is reached anyway.
  */
 #include <hip/hip_runtime.h>
