@@ -17,7 +17,7 @@
  * the converged state takes them), applied in transposed form by a kernel
  * that recomputes every coefficient and its derivatives from the stored
  * converged state.  The solve is GMRES on the finest level, preconditioned
- * by two V-cycles of the same multigrid hierarchy: on every level an adjoint
+ * by V-cycles of the same multigrid hierarchy: on every level an adjoint
  * SIMPLE relaxation (red-black sweeps of the transposed scalar and momentum
  * operators, and the pressure-correction equation for the adjoint
  * continuity multiplier), the reference's restriction weights for the
@@ -56,6 +56,8 @@ typedef struct mgf_adjoint_opts {
     int single_colour; /* 1: the preconditioner's red-black sweeps as one launch per
                           colour and sweep instead of several sweeps per launch
                           (default 0); the same result bit for bit, for checking */
+    int precond_cycles;/* V-cycles per use of the preconditioner, 1..4; 0 (default):
+                          chosen by the batch size */
 } mgf_adjoint_opts;
 
 void mgf_adjoint_opts_default(mgf_adjoint_opts *o);
@@ -97,8 +99,8 @@ int mgf_adjoint_get_state(mgf_solver *s, int b, double *lam_u, double *lam_v, do
  * the current state, on caller vectors of 4 * npx * npy doubles (level 0:
  * the u, v, continuity and scalar parts, each npx*npy).  op 0: out =
  * K^T in (the exact transposed Jacobian); op 1: out = the
- * preconditioner (two cycles of the adjoint V-cycle iteration from zero)
- * applied to the right-hand side in. */
+ * preconditioner (the default number of cycles of the adjoint V-cycle
+ * iteration from zero) applied to the right-hand side in. */
 int mgf_adjoint_apply(mgf_solver *s, int b, int op, const double *in, double *out);
 
 #ifdef __cplusplus

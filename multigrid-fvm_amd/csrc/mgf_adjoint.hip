@@ -1137,6 +1137,7 @@ void **mgf_adjoint_slot(mgf_solver *s);
 struct AdjWork {
     int nb = 0, restart = 0;
     int single_colour = 0;                          /* sweeps as single-colour launches */
+    int pc_cycles = 1;                              /* V-cycles per preconditioner use */
     std::vector<int> ng;
     std::vector<std::vector<AdjLevel>> lv;          /* host copies per problem and level */
     std::vector<AdjVec> vec;
@@ -1500,17 +1501,22 @@ static void vcycle(AdjWork *w, int nswm, int nswp, hipStream_t st, bool first = 
     }
 }
 
-/* the preconditioner: ADJ_PRECOND_CYCLES cycles of the V-cycle iteration
- * from e = 0 (a fixed linear operator on f).  On the 64 x 512^2 batch, two
- * cycles take 35 GMRES iterations where one takes 50, and the whole adjoint
- * 8% less time (the Gram-Schmidt cost grows with the iterations squared);
- * three take 30 and more time. */
-#ifndef ADJ_PRECOND_CYCLES
-#define ADJ_PRECOND_CYCLES 2
+/* precond_cycles 0: the V-cycles per preconditioner use for a batch of nb
+ * problems.  A second cycle cuts the GMRES iterations by about a third and
+ * costs a V-cycle per iteration; it pays where the batch holds problems that
+ * need many iterations, whose Gram-Schmidt cost grows with the iterations
+ * squared.  Measured (512^2 adjoint-workload problems, restart 60; one
+ * cycle against two): 2, 4, 8 problems 0.062/0.089, 0.110/0.124,
+ * 0.151/0.170 s; 16, 32, 64 problems 0.491/0.466, 0.798/0.745, 1.44/1.32 s;
+ * the 256^2 single problem 0.062/0.077 s. */
+#ifndef ADJ_AUTO_CYCLES
+#define ADJ_AUTO_CYCLES(nb) ((nb) >= 16 ? 2 : 1)
 #endif
+/* the preconditioner: w->pc_cycles cycles of the V-cycle iteration from
+ * e = 0 (a fixed linear operator on f) */
 static void precondition(AdjWork *w, int nswm, int nswp, hipStream_t st)
 {
-    for (int c = 0; c < ADJ_PRECOND_CYCLES; ++c) vcycle(w, nswm, nswp, st, c == 0);
+    for (int c = 0; c < w->pc_cycles; ++c) vcycle(w, nswm, nswp, st, c == 0);
 }
 
 /* ---- pieces of the solve ---------------------------------------------- */
@@ -1671,6 +1677,7 @@ extern "C" void mgf_adjoint_opts_default(mgf_adjoint_opts *o)
     o->restart = 60;
     o->only_converged = 1;
     o->single_colour = 0;
+    o->precond_cycles = 0;
 }
 
 /* Givens rotation of (a, b) */
@@ -1696,12 +1703,14 @@ extern "C" int mgf_batch_adjoint(mgf_solver *s, int objective, const mgf_adjoint
     mgf_adjoint_opts o;
     mgf_adjoint_opts_default(&o);
     if (opt) o = *opt;
-    if (o.restart < 1 || o.restart > 200 || o.max_iter < 1 || !(o.rtol > 0.0))
+    if (o.restart < 1 || o.restart > 200 || o.max_iter < 1 || !(o.rtol > 0.0) ||
+        o.precond_cycles < 0 || o.precond_cycles > 4)
         return mgf_set_error(s, MGF_ERR_ARG, "bad adjoint options");
     AdjWork *w = nullptr;
     int rc = prepare(s, o.restart, &w);
     if (rc) return rc;
     w->single_colour = o.single_colour != 0;
+    w->pc_cycles = o.precond_cycles > 0 ? o.precond_cycles : ADJ_AUTO_CYCLES(w->nb);
     hipStream_t st = (hipStream_t)mgf_stream(s);
     const int nb = w->nb, m = o.restart;
     int nswm, nswp;
@@ -2062,6 +2071,7 @@ extern "C" int mgf_adjoint_apply(mgf_solver *s, int b, int op, const double *in,
     int rc = prepare(s, w ? w->restart : o.restart, &w);
     if (rc) return rc;
     w->single_colour = 0;
+    w->pc_cycles = ADJ_AUTO_CYCLES(w->nb);
     hipStream_t st = (hipStream_t)mgf_stream(s);
     for (int q = 0; q < w->nb; ++q) w->h_live[q] = q == b;
     if ((rc = upload_live(s, w, st))) return rc;

@@ -28,7 +28,8 @@ OBJ_OUTLET_FLUX = 1
 
 class AdjointOpts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("max_iter", ctypes.c_int), ("restart", ctypes.c_int),
-                ("only_converged", ctypes.c_int), ("single_colour", ctypes.c_int)]
+                ("only_converged", ctypes.c_int), ("single_colour", ctypes.c_int),
+                ("precond_cycles", ctypes.c_int)]
 
 
 _bound = False
@@ -60,7 +61,8 @@ def _check(h, rc, allow_nonfinite=False):
 
 def opts(rtol: Optional[float] = None, max_iter: Optional[int] = None,
          restart: Optional[int] = None, only_converged: Optional[bool] = None,
-         single_colour: Optional[bool] = None) -> AdjointOpts:
+         single_colour: Optional[bool] = None,
+         precond_cycles: Optional[int] = None) -> AdjointOpts:
     o = AdjointOpts()
     _bind().mgf_adjoint_opts_default(ctypes.byref(o))
     if rtol is not None:
@@ -73,6 +75,8 @@ def opts(rtol: Optional[float] = None, max_iter: Optional[int] = None,
         o.only_converged = int(only_converged)
     if single_colour is not None:
         o.single_colour = int(single_colour)
+    if precond_cycles is not None:
+        o.precond_cycles = int(precond_cycles)
     return o
 
 
