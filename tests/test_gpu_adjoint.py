@@ -263,6 +263,34 @@ def test_batched_adjoint_workload_reduced(tmp_path):
         _close(resm[q][2], res[k][2], 1e-5, f"mixed amuf {k}")
 
 
+def test_batched_adjoint_mixed_shapes_equals_singles(tmp_path):
+    """A batch whose problems differ in size, depth and obstacle level (32^2
+    with 3 levels twice, at two viscosities, and 64 x 32 with obstacles
+    coarsened below their level): every member's batched gradients of both
+    objectives equal those of the same problem solved and differentiated
+    alone, to 1e-12."""
+    import mgfvm
+    from mgfvm import adjoint as gadj
+    (p1, f1), (p3, f3) = _case32(), _case64_coarse_obstacles()
+    cases = [(p1, f1), (p1.replace(amuf=0.8 * p1.amuf), f1), (p3, f3)]
+    pairs = [p.write(str(tmp_path / f"m{k}")) for k, (p, _) in enumerate(cases)]
+    _say("[gpu] batched adjoint over mixed shapes vs single problems")
+    with mgfvm.Batch(pairs) as bt:
+        for b, (_, prof) in enumerate(cases):
+            bt.set_inlet_profile(b, prof)
+        nc, cv = bt.solve()
+        assert all(cv)
+        res = {obj: gadj.batch_adjoint(bt, obj) for obj in (0, 1)}
+    for b, ((pin, ppat), (_, prof)) in enumerate(zip(pairs, cases)):
+        with mgfvm.Solver(pin, ppat, inlet_u=prof) as s:
+            assert s.solve()[2]
+            for obj in (0, 1):
+                gi, ga, gp = gadj.adjoint(s, obj)
+                _close(res[obj][b][1], gi, 1e-12, f"member {b} obj {obj} inlet rows")
+                _close(res[obj][b][2], ga, 1e-12, f"member {b} obj {obj} amuf")
+                _close(res[obj][b][3], gp, 1e-12, f"member {b} obj {obj} prsc")
+
+
 def test_blocked_sweeps_equal_single_colour(tmp_path):
     """The preconditioner's sweeps run several per launch on LDS tiles (two
     launches per field, to a scratch array and back); the same sweeps as
