@@ -335,6 +335,32 @@ def test_unconverged_solver_is_refused(tmp_path):
             gadj.adjoint(s, 2)
 
 
+def test_discrete_inlets_and_outlets_match_forward_mode(tmp_path):
+    """Discrete inlets and outlets (on level 0) set fixed ghost values, which
+    the adjoint's rows model: on the 32^2 case with an inlet (its sc 1) and
+    an outlet over part of the east face, every direction of both objectives
+    equals the forward-mode library's."""
+    import mgfvm
+    from mgfvm import adjoint as gadj
+    from mgfvm.tangent import gradient_c
+    prob, prof = _case32()
+    um = float(prof.mean())
+    prob = prob.replace(ig_inlt=1, ig_outlt=1, inlets=[(1, 32, 1.0, 0.0, 1.0)],
+                        outlets=[(9, 24, um, 0.0, 0.0)])
+    pin, ppat = prob.write(str(tmp_path / "io"))
+    _say("[gpu] discrete inlets/outlets: forward mode vs adjoint")
+    J, gin, gmu, gpr, conv = gradient_c(pin, ppat, inlet_u=prof)
+    assert conv
+    with mgfvm.Solver(pin, ppat, inlet_u=prof) as s:
+        assert s.solve()[2]
+        for obj in (0, 1):
+            gi, ga, gp = gadj.adjoint(s, obj)
+            _close(gi, gin[obj], GRAD_RTOL, f"obj {obj} inlet rows")
+            _close(ga, gmu[obj], GRAD_RTOL, f"obj {obj} amuf")
+            if obj == 1:
+                _close(gp, gpr[obj], GRAD_RTOL, "obj 1 prsc")
+
+
 def test_unmodelled_boundaries_are_refused(tmp_path):
     """The adjoint's rows hold every ghost value fixed (ibc 1): a problem
     with a p' Dirichlet row, periodic or outflow side is refused with
