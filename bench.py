@@ -224,6 +224,12 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
                                        "a stored equation (split assembly, or p' beyond six "
                                        "sweeps), same members",
                              "mean_us": round(us_s, 2), "bytes_per_launch": nbytes_s,
+                             "compulsory_bytes_per_launch": 32 * ncell,
+                             "compulsory_frac": round(32 * ncell / (us_s * 1e-6) / 1e9
+                                                      / HBM_PEAK_GBS, 4),
+                             "compulsory_note": "the equation (6 FP32 words) and d read once "
+                                                "and d written once per launch, against the "
+                                                "HBM peak",
                              "per_sweep_effective_GBps": round(nbytes_s / (us_s * 1e-6) / 1e9, 1),
                              "note": "8 words per unknown and sweep over 3 temporally blocked "
                                      "sweeps whose data stay on chip: an effective rate, not "
