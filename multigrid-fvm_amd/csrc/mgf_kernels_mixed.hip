@@ -336,15 +336,16 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
     unsigned meta[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
+        /* pt[k] and apk[k] are set only for points inside the swept range
+         * (meta bit 9) and read only for those: no initial value, which the
+         * compiler would otherwise materialise in every branch */
         const int p = tid + k * NT;
         float val = 0.f;
         unsigned m = 0;
-        pt[k] = Pt32{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (ASM) apk[k] = 0.0;
-        if (p < NR) {
-            const int r = p / RW, c = p - r * RW;
-            const int gi = ri0 + c, gj = rj0 + r;
-            if (gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1) {
+        const int r = p / RW, c = p - r * RW;
+        const int gi = ri0 + c, gj = rj0 + r;
+        {
+            if (p < NR && gi >= 1 && gi <= nx - 1 && gj >= 1 && gj <= ny - 1) {
                 const int g = gj * npx + gi;
                 const int d = min(min(c, RW - 1 - c), min(r, RH - 1 - r));
                 if (ASM) {
@@ -367,7 +368,7 @@ __device__ __forceinline__ void rb32x_tile(const DevLevelG &L, const TileId T, i
                 m |= (unsigned)d | ((unsigned)((gi + gj) & 1) << 8) | (1u << 9);
                 if (stash && d >= H && (L.flags[g] & MGF_F_PORIG)) m |= 1u << 10;
             }
-            t[p] = val;
+            if (p < NR) t[p] = val;
         }
         meta[k] = m;
     }
