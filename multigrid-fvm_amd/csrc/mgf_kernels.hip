@@ -1226,9 +1226,11 @@ __device__ __forceinline__ void rb_sweeps_tile(const DevLevelG &L, const TileId 
     unsigned meta[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
+        /* own[k] is set only for points inside the swept range (meta bit 9)
+         * and read only for those: no initial value, which the compiler would
+         * otherwise materialise in every branch */
         const int p = tid + k * RBX_NT;
         unsigned m = 0;
-        own[k] = Pt{0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
         if (p < NR) {
             const int r = p / RW, c = p - r * RW;
             const int gi = ri0 + c, gj = rj0 + r;
