@@ -36,7 +36,10 @@ size measured under that load on bounded runs, times the cycle counts of
 the batch; the one-core figure beside it).
 
 --roofline-only runs just the dominant kernel's launches and a 1 GiB device
-copy, for the rocprofv3 counter passes that give `traffic`.
+copy, for the rocprofv3 counter passes that give `traffic`.  --tail-scaling
+times V-cycles of 1 .. 32 of the 512^2 problems at Re 10/50 solved together
+(the fixed latency chain and the per-problem part of the non-converging
+tail that sets the step).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -307,6 +310,59 @@ def roofline_only(args):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def tail_scaling(args):
+    """V-cycle time of n of the batch's 512^2 problems at Re 10/50 (the set
+    whose 24 non-converging members set the step), solved together for a
+    fixed number of V-cycles, for n = 1 .. 32 with 1 and 3 lanes: the split
+    of the tail's V-cycle into a fixed latency chain and a per-problem part.
+    Prints one JSON line."""
+    import torch
+    import mgfvm
+    from mgfvm.inputs import staggered_batch
+    torch.cuda.set_device(0)
+    ncyc = 100
+    work = tempfile.mkdtemp(prefix="mgf_tail_")
+    try:
+        probs = [p.replace(nvcycle=ncyc, convergence=0.0) for p in staggered_batch(args.count)
+                 if p.nfx == 512 and p.amuf >= 1.0 / 50 - 1e-12]
+        pairs = [p.write(os.path.join(work, f"t{k}")) for k, p in enumerate(probs)]
+        rows = []
+        for n in (1, 2, 4, 8, 16, 24, 32):
+            if n > len(pairs):
+                break
+            for lanes in (1, 3):
+                if lanes > n:
+                    continue
+                b = mgfvm.Batch(pairs[:n], precision=mgfvm.PREC_MIXED, device=0, lanes=lanes)
+                b.solve()
+                b.reset()
+                _sync(b)
+                t0 = time.perf_counter()
+                b.solve()
+                _sync(b)
+                dt = time.perf_counter() - t0
+                b.close()
+                rows.append({"problems": n, "lanes": lanes,
+                             "ms_per_v_cycle": round(dt / ncyc * 1e3, 4)})
+                print(f"[bench] tail: {n} problems, {lanes} lanes: {dt / ncyc * 1e3:.3f} ms "
+                      "per V-cycle", file=sys.stderr, flush=True)
+        # least squares of ms per V-cycle against n, 3 lanes, n >= 4
+        pts = [(r["problems"], r["ms_per_v_cycle"]) for r in rows
+               if r["lanes"] == 3 and r["problems"] >= 4]
+        fit = None
+        if len(pts) >= 2:
+            mx = sum(x for x, _ in pts) / len(pts)
+            my = sum(y for _, y in pts) / len(pts)
+            slope = (sum((x - mx) * (y - my) for x, y in pts) /
+                     sum((x - mx) ** 2 for x, _ in pts))
+            fit = {"fixed_ms": round(my - slope * mx, 4), "per_problem_us": round(slope * 1e3, 2)}
+        print(json.dumps({"workload": "512^2 problems of the batch at Re 10 and 50, mixed "
+                                      f"precision, {ncyc} V-cycles from the initial state",
+                          "rows": rows, "fit_3_lanes_n_ge_4": fit}))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 # GMRES restart of the batched-adjoint figure: 100 Krylov vectors (plus 4
 # work vectors) of 4 x 514^2 doubles per 512^2 problem, 0.88 GB each
 ADJ_RESTART = 100
@@ -383,6 +439,9 @@ def main():
                     help="run only the batched-adjoint figure (for kernel traces of that path)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="solve only the batch in FP64, once (for kernel traces of that path)")
+    ap.add_argument("--tail-scaling", action="store_true",
+                    help="time V-cycles of 1 .. 32 of the 512^2 problems at Re 10/50 together "
+                         "(the fixed and per-problem parts of the non-converging tail)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
@@ -400,6 +459,9 @@ def main():
         sys.exit(1)
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.tail_scaling:
+        tail_scaling(args)
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_figure(args.adjoint_count, tempfile.mkdtemp(prefix="mgf_adj_"))))
