@@ -196,7 +196,7 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
     batch's own stream over every member of the largest grid size at level 0,
     after that batch has run (graph-replayed V-cycles cannot be bracketed per
     kernel, so the launch is repeated on its own).  Also reported: the FP32
-    sweep kernel (k_rb32x<3,512,0,-1>, three p' sweeps on the stored
+    sweep kernel (k_rb32x<3,1024,0,-1>, three p' sweeps on the stored
     equation)."""
     import mgfvm
     nmax = max(p.nfx for p in probs)
@@ -223,7 +223,7 @@ def kernel_roofline(probs, pairs, reps, cycles=None):
                       f"members, level 0",
             "mean_us": round(us, 2), "bytes_per_launch": nbytes,
             "bytes_per_cell": round(nbytes / ncell, 2),
-            "sweep_kernel": {"kernel": "k_rb32x<3,512,0,-1>: 3 FP32 red-black p' sweeps on "
+            "sweep_kernel": {"kernel": "k_rb32x<3,1024,0,-1>: 3 FP32 red-black p' sweeps on "
                                        "a stored equation (split assembly, or p' beyond six "
                                        "sweeps), same members",
                              "mean_us": round(us_s, 2), "bytes_per_launch": nbytes_s,
