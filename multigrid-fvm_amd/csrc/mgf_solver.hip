@@ -1325,9 +1325,10 @@ static int build_lanes(mgf_solver *s, const std::vector<int> &alive, int nl)
  * Compaction: launches over stopped instances still cost their dispatch
  * (and a group whose members have all stopped, its launch latency), which
  * in a batch whose slowest members run far longer than the rest is a
- * sizeable share of the tail.  When at most 3/4 of the instances of the
+ * sizeable share of the tail.  When at most 7/8 of the instances of the
  * running schedule are still live, the lanes are rebuilt over the live ones
- * alone and replace the old ones from the next chunk on.
+ * alone and replace the old ones from the next chunk on (7/8: the benchmark
+ * batch's step 1% shorter than with 3/4 or 9/10, 2% shorter than with 1/2).
  *
  * Every instance's arithmetic is its own, so neither changes the results. */
 static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
@@ -1390,7 +1391,7 @@ static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
             for (int b : P->members)
                 if (!s->h_ctl[b].done) alive.push_back(b);
         if (alive.empty()) break;
-        if (s->opt.use_graph && s->opt.compact && 4 * alive.size() <= 3 * scheduled) {
+        if (s->opt.use_graph && s->opt.compact && 8 * alive.size() <= 7 * scheduled) {
             std::sort(alive.begin(), alive.end());
             nl = lane_count(s, alive.size());
             if ((rc = ensure_lanes(s, nl)) || (rc = build_lanes(s, alive, nl))) return rc;
