@@ -1399,6 +1399,8 @@ static int run_cycles(mgf_solver *s, const int *limit, const double *tol)
             for (Program &P : s->lanes) run.push_back(&P);
             scheduled = alive.size();
         }
+        /* chunks of up to 32 cycles between polls: the benchmark batch's step
+         * was 0.2-2% longer with caps of 8, 16, 64 or 128 */
         if (chunk < 32) chunk *= 2;
     }
     return read_ctl(s);
