@@ -257,7 +257,8 @@ __device__ __forceinline__ void extents(const DevLevelG &L, int field, int &nx, 
 #endif
 /* tile rows of the launches on a stored equation: 32 (halo 1.63x the tile
  * at three sweeps, against 2.08x on 16 rows) measured 2% faster on 84
- * members of 512 x 512 than 16; 24 and 48 were slower */
+ * members of 512 x 512 than 16 with 512 threads, and 24 and 48 slower; with
+ * 1024 threads 48 and 64 rows were slower as well */
 #ifndef MGF_LOAD_TILE_ROWS
 #define MGF_LOAD_TILE_ROWS 32
 #endif
